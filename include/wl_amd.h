@@ -1,0 +1,231 @@
+/*
+ * C ABI of libwl_amd: a HIP/gfx950 replica-exchange Wang-Landau engine for
+ * 2D random-bond Ising models.
+ *
+ * Part 1 mirrors the reference's kernel library (src/wl.cuh): one entry point
+ * per kernel, same parameter order and meaning.  The reference's callers pick
+ * the launch shape with <<<grid, block>>>; here the library picks it, so the
+ * shape parameters the caller used to encode in the launch (walkers per
+ * interval, number of samples) are explicit arguments, and every call takes
+ * the HIP stream (NULL = default stream) it is enqueued on.  All pointers are
+ * device pointers owned by the caller; nothing here allocates except where
+ * stated.
+ *
+ * Part 2 adds the drivers around the kernels: the mainrun loop of the
+ * reference (src/mainrun.cu:243-383) on device state, the batched persistent
+ * form, stitching and log Z.
+ *
+ * Every function returns 0 on success, 1 for an invalid argument and 2 for a
+ * HIP/runtime failure; wl_last_error() gives the message.  No function
+ * throws across this boundary.
+ *
+ * Types: the reference's `unsigned long` counters are uint64_t, its
+ * `unsigned long long` histograms are uint64_t, `signed char`/`int8_t`
+ * spins and couplings are int8_t, and wl::boundary is an int (0 = periodic,
+ * 1 = open; src/boundary.hpp:10-14).
+ */
+#ifndef WL_AMD_H_
+#define WL_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* wl_last_error(void);
+
+/* Library version string written into result files. */
+const char* wl_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Part 1: kernel library (replaces src/wl.cuh:24-514)                 */
+/* ------------------------------------------------------------------ */
+
+/* calc_energy, src/wl.cuh:70-80 (periodic/open kernels wl.cuh:24-52).
+ * Energy of every lattice; lattice t uses the couplings of sample
+ * t / walker_per_interactions. */
+int wl_calc_energy(int boundary, int* d_energy, const int8_t* d_lattice, const int8_t* d_interactions,
+                   const int* d_offset_lattice, int dim_X, int dim_Y, size_t num_lattices,
+                   int walker_per_interactions, void* stream);
+
+/* init_interactions, src/wl.cuh:95-99: J = -1 with probability prob, one
+ * draw of stream (seed + sample, disorder tag) per bond. */
+int wl_init_interactions(int8_t* d_interactions, size_t lattice_size, int num_interactions, int seed, float prob,
+                         void* stream);
+
+/* apply_x_horizontal_error, src/wl.cuh:113-116: flip the down bonds of row 0. */
+int wl_apply_x_horizontal_error(int8_t* d_interactions, int lattice_dim_x, int lattice_dim_y, int num_interactions,
+                                void* stream);
+
+/* apply_x_vertical_error, src/wl.cuh:130-133: flip the right bonds of column 0. */
+int wl_apply_x_vertical_error(int8_t* d_interactions, int lattice_dim_x, int lattice_dim_y, int num_interactions,
+                              void* stream);
+
+/* init_lattice, src/wl.cuh:150-156.  Lattice l of sample s draws its spin
+ * probability q from its own stream slot (the reference publishes q through a
+ * cross-block race, src/wl.cu:450-454); q is stored in d_init_probs. */
+int wl_init_lattice(int8_t* d_lattice, float* d_init_probs_per_lattice, int dim_X, int dim_Y, size_t num_lattices,
+                    int seed, int walker_per_interactions, void* stream);
+
+/* init_offsets_lattice, src/wl.cuh:170-173: off[t] = t * X * Y. */
+int wl_init_offsets_lattice(int* d_offset_lattice, int lattice_dim_x, int lattice_dim_y, size_t num_lattices,
+                            void* stream);
+
+/* wang_landau_pre_run, src/wl.cuh:200-218.  One workgroup per sample
+ * (walker_per_interactions <= 1024).  Walkers of a sample move in lock step:
+ * each step's decisions read the shared histogram as it stood at the start
+ * of the step, so the result is deterministic (the reference's version is
+ * racy by design, src/wl.cu:534-556).  The lowest-index walker that accepts
+ * into an empty prerun interval stores its lattice. */
+int wl_wang_landau_pre_run(int* d_energy, int* d_flag_found_interval, int8_t* d_lattice,
+                           int8_t* d_found_interval_lattice, uint64_t* d_histogram, uint64_t* d_seed_offset,
+                           const int8_t* d_interactions, const int* d_offset_lattice_per_walker, int E_min, int E_max,
+                           int num_wl_iterations, int dim_X, int dim_Y, int seed, unsigned int num_intervals,
+                           int len_interval, size_t num_total_walker, int walker_per_interactions, int boundary,
+                           void* stream);
+
+/* check_energy_ranges, src/wl.cuh:233-237 (interval = walker / walker_per_interval). */
+int wl_check_energy_ranges(int8_t* d_flag_check_energies, const int* d_energy_per_walker,
+                           const int* d_interval_start_energies, const int* d_interval_end_energies,
+                           size_t num_total_walker, int walker_per_interval, void* stream);
+
+/* init_indices, src/wl.cuh:248: idx[t] = t % walker_per_interval. */
+int wl_init_indices(int* d_replica_indices, size_t num_total_walker, int walker_per_interval, void* stream);
+
+/* reset_d_cond, src/wl.cuh:261-265. */
+int wl_reset_d_cond(int8_t* d_cond_interval, const double* d_factor, int total_intervals, double beta,
+                    int walker_per_interval, void* stream);
+
+/* init_offsets_histogram, src/wl.cuh:279-284. */
+int wl_init_offsets_histogram(int* d_offset_histogram, const int* d_interval_start_energies,
+                              const int* d_interval_end_energies, const int* d_len_histograms, int num_intervals,
+                              int total_walker, int walker_per_interval, void* stream);
+
+/* calc_average_log_g, src/wl.cuh:306-320.  Adds, for every spectrum bin of
+ * every interval with cond == 1 of an unfinished sample, logG[w][e] / W for
+ * w = 0..W-1 in that order onto d_shared_logG (the reference sums with FP64
+ * atomics in an unspecified order, src/wl.cu:779). */
+int wl_calc_average_log_g(double* d_shared_logG, const int* d_len_histograms, const double* d_log_G,
+                          const int* d_interval_start_energies, const int* d_interval_end_energies,
+                          const int8_t* d_expected_energy_spectrum, const int8_t* d_cond,
+                          const int* d_offset_histogram, const int* d_offset_energy_spectrum,
+                          const long long* d_offset_shared_logG, const int* d_cond_interaction, int num_interactions,
+                          int num_walker_per_interval, int num_intervals_per_interaction, int total_len_histogram,
+                          void* stream);
+
+/* check_interactions_finished, src/wl.cuh:333-338.  cond_sample[s] = sum of
+ * cond over the sample's intervals, -1 if all are 1.  Needs no scratch
+ * buffer (the reference's CUB reduction takes a device_tmp). */
+int wl_check_interactions_finished(int* d_cond_interactions, const int8_t* d_cond_interval,
+                                   const int* d_offset_intervals, int num_intervals, int num_interactions,
+                                   void* stream);
+
+/* replica_exchange, src/wl.cuh:357-369.  `flags` bit 0: pair intervals by
+ * the global interval index parity as the reference does (src/wl.cu:830);
+ * otherwise by the index within the sample.  Acceptance is
+ * min(1, exp((g_i(E_X)-g_i(E_Y)) + (g_j(E_Y)-g_j(E_X)))). */
+int wl_replica_exchange(int* d_offset_lattice, int* d_energy, int* d_replica_indices, uint64_t* d_seed_offset,
+                        const int* d_interval_start_energies, const int* d_interval_end_energies,
+                        const int* d_offset_histogram, const int* d_cond_interaction, const double* d_logG, int even,
+                        int seed, int num_intervals, int walker_per_interactions, int num_interactions, int flags,
+                        void* stream);
+
+/* check_histogram, src/wl.cuh:389-401 (note: end before start). */
+int wl_check_histogram(uint64_t* d_histogram, double* d_factor, int8_t* d_cond_interval,
+                       const int* d_offset_histogram, const int* d_interval_end_energies,
+                       const int* d_interval_start_energies, const int8_t* d_expected_energy_spectrum,
+                       const int* d_offset_energy_spectrum, const int* d_cond_interaction, double alpha,
+                       int num_walker_total, int walker_per_interactions, int num_intervals, void* stream);
+
+/* redistribute_g_values, src/wl.cuh:421-433. */
+int wl_redistribute_g_values(double* d_log_G, const double* d_shared_logG, const int* d_len_histograms,
+                             const int* d_interval_start_energies, const int* d_interval_end_energies,
+                             const int* d_cond_interaction, const int* d_offset_histogram,
+                             const int8_t* d_cond_interval, const long long* d_offset_shared_logG,
+                             int num_intervals_per_interaction, int num_walker_per_interval, int num_interactions,
+                             int total_len_histogram, void* stream);
+
+/* wang_landau, src/wl.cuh:466-490.  The walker sweep.  Energies outside the
+ * sample's [E_min, E_max] count as new energies (the reference reads out of
+ * bounds there, src/wl.cu:1091), and converged intervals only move inside
+ * their range (src/wl.cu:1161 uses ||). */
+int wl_wang_landau(int8_t* d_lattice, const int8_t* d_interactions, uint64_t* d_H, double* d_logG,
+                   const double* factor, int* d_energy, int* d_newEnergies, uint64_t* d_offset_iter, int* foundFlag,
+                   const int* d_start, const int* d_end, const int* d_offset_histogram, const int* d_offset_lattice,
+                   const int8_t* d_cond, const int8_t* d_expected_energy_spectrum,
+                   const int* d_offset_energy_spectrum, const int* d_cond_interaction, int dimX, int dimY,
+                   int num_iterations, int num_lattices, int num_intervals, int walker_per_interactions, int seed,
+                   int boundary, void* stream);
+
+/* fisher_yates, src/wl.cuh:510-514 (a __device__ function there): every
+ * interval's walker 0 shuffles that interval's index segment. */
+int wl_fisher_yates(int* d_shuffle, uint64_t* d_seed_offset, int seed, int total_intervals, int walker_per_interval,
+                    int walker_per_interactions, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Part 2: drivers                                                      */
+/* ------------------------------------------------------------------ */
+
+/* Reference mainrun loop, src/mainrun.cu:243-383, on device state laid out as
+ * there (S samples, I intervals, W walkers).  Runs until every factor is
+ * within 1e-10 of exp(beta), a "new energy" abort (returns 3, *rounds_out is
+ * the round), or max_rounds.  exchange_every = -c; flags bit 0 = reference
+ * compatibility (exchange only when c == 1, global parity).  d_idx is the
+ * replica index array, d_cond_sample the per-sample flags. */
+int wl_mainrun_loop(int8_t* d_lattice, const int8_t* d_interactions, uint64_t* d_H, double* d_logG,
+                    double* d_factor, int* d_energy, uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx,
+                    int8_t* d_cond, int* d_cond_sample, const int* d_start, const int* d_end,
+                    const int* d_offset_histogram, const int8_t* d_spectrum, const int* d_offset_spectrum,
+                    const int* d_len_histograms, int S, int I, int W, int X, int Y, int boundary, int num_iterations,
+                    double alpha, double beta, int exchange_every, int seed, int flags, int64_t max_rounds,
+                    int64_t* counter_io, int64_t* rounds_out, void* stream);
+
+/* One (disorder sample, error class) task of the batched form.  Its state
+ * lives in the caller's flat arrays in the reference's single-sample layout
+ * (S = 1) starting at the given offsets; lattice offsets in
+ * d_offset_lattice are absolute byte offsets into d_lattice. */
+typedef struct wl_task {
+  int X, Y, boundary;     /* lattice */
+  int I, W;               /* intervals, walkers per interval */
+  int rng_key;            /* seed + sample: first key word of the walker streams */
+  int num_iterations;     /* WL attempts per walker per round (-n) */
+  int exchange_every;     /* -c */
+  int energy_stride;      /* spacing of reachable energies: 4 periodic, 2 open */
+  int parity_base;        /* global interval index of interval 0 (compat parity) */
+  int flags;              /* bit 0: reference compatibility */
+  int interval_off;       /* first entry in start/end/cond */
+  int spectrum_off;       /* first byte of the mask over [start[0], end[I-1]] */
+  int pad_;
+  int64_t walker_off;     /* first walker: E, counters, factor, idx, offsets */
+  int64_t hist_off;       /* first element of the task's logG / H block */
+  int64_t lattice_off;    /* first byte of the task's lattice block (I*W*X*Y) */
+  int64_t couplings_off;  /* first byte of the task's couplings (2*X*Y) */
+  double alpha, beta;
+} wl_task;
+
+/* Bytes of LDS the fused kernel needs for a task whose interval table is
+ * h_start/h_end (host arrays of task->I entries); 0 if it cannot run it. */
+size_t wl_fused_task_lds_bytes(const wl_task* task, const int* h_start, const int* h_end);
+
+/* Batched persistent form: one workgroup runs a whole task (every interval
+ * and walker) for up to max_rounds rounds of sweep + flatness + averaging +
+ * exchange, with lattices bit-packed and ln g / histograms compacted to the
+ * reachable energies in LDS; workgroups pull tasks from a device queue.
+ * d_tasks is a device copy of the task table.  Per task: d_counter is the
+ * exchange counter (start at 1), d_rounds accumulates rounds, d_status is
+ * 0 = running, 1 = finished, 2 = new energy, 3 = too large for one workgroup.
+ * Bit-identical to wl_mainrun_loop on the same task with the same rounds. */
+int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, int8_t* d_lattice,
+                 const int8_t* d_interactions, uint64_t* d_H, double* d_logG, double* d_factor, int* d_energy,
+                 uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx, int8_t* d_cond, int* d_cond_sample,
+                 const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
+                 int64_t* d_rounds, int* d_status, int64_t max_rounds, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WL_AMD_H_ */

@@ -1,0 +1,98 @@
+"""ctypes binding of libwl_amd.so (built in-tree by ``qec_pf_wang_landau_amd.build``).
+
+Loading fails loudly when the library is missing: there is no CPU fallback
+for any entry point.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libwl_amd.so"
+
+P = C.c_void_p
+I32 = C.c_int
+U32 = C.c_uint
+I64 = C.c_int64
+SZ = C.c_size_t
+F32 = C.c_float
+F64 = C.c_double
+
+# name -> (restype, argtypes); the order follows include/wl_amd.h and include/wl_host.h.
+SIGNATURES: dict[str, tuple[object, list[object]]] = {
+    "wl_last_error": (C.c_char_p, []),
+    "wl_version": (C.c_char_p, []),
+    "wl_calc_energy": (I32, [I32, P, P, P, P, I32, I32, SZ, I32, P]),
+    "wl_init_interactions": (I32, [P, SZ, I32, I32, F32, P]),
+    "wl_apply_x_horizontal_error": (I32, [P, I32, I32, I32, P]),
+    "wl_apply_x_vertical_error": (I32, [P, I32, I32, I32, P]),
+    "wl_init_lattice": (I32, [P, P, I32, I32, SZ, I32, I32, P]),
+    "wl_init_offsets_lattice": (I32, [P, I32, I32, SZ, P]),
+    "wl_wang_landau_pre_run": (I32, [P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, U32, I32, SZ, I32, I32, P]),
+    "wl_check_energy_ranges": (I32, [P, P, P, P, SZ, I32, P]),
+    "wl_init_indices": (I32, [P, SZ, I32, P]),
+    "wl_reset_d_cond": (I32, [P, P, I32, F64, I32, P]),
+    "wl_init_offsets_histogram": (I32, [P, P, P, P, I32, I32, I32, P]),
+    "wl_calc_average_log_g": (I32, [P, P, P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, P]),
+    "wl_check_interactions_finished": (I32, [P, P, P, I32, I32, P]),
+    "wl_replica_exchange": (I32, [P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, P]),
+    "wl_check_histogram": (I32, [P, P, P, P, P, P, P, P, P, F64, I32, I32, I32, P]),
+    "wl_redistribute_g_values": (I32, [P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, P]),
+    "wl_wang_landau": (I32, [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                             I32, I32, I32, I32, I32, I32, I32, I32, P]),
+    "wl_fisher_yates": (I32, [P, P, I32, I32, I32, I32, P]),
+    "wl_mainrun_loop": (I32, [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                              I32, I32, I32, I32, I32, I32, I32, F64, F64, I32, I32, I32, I64, P, P, P]),
+    "wl_fused_task_lds_bytes": (SZ, [P, P, P]),
+    "wl_fused_run": (I32, [P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, P]),
+    # host side (include/wl_host.h)
+    "wl_interval_results": (I32, [I32, I32, U32, I32, F32, P, P, P, P]),
+    "wl_energy_spectrum": (I32, [P, I32, I32, I32, P, P, P, P]),
+    "wl_energy_stride": (I32, [P, I32]),
+    "wl_select_seed_lattices": (I32, [P, P, I32, I32, P, P, I32, I32, P]),
+    "wl_stitch_log_g": (I32, [P, P, P, I32, I32, I32, I32, P, P, I32, P]),
+    "wl_log_partition": (F64, [P, P, I32, F64]),
+    "wl_nishimori_beta": (F64, [F64]),
+}
+
+
+class WLError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, name: str, status: int, message: str):
+        super().__init__(f"{name} failed with status {status}: {message}")
+        self.status = status
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """Load (once) and return the library with all signatures declared."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m qec_pf_wang_landau_amd.build` "
+                "(there is no CPU fallback)")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def call(name: str, *args) -> int:
+    """Call an int-status entry point and raise WLError on failure."""
+    fn = getattr(lib(), name)
+    rc = fn(*args)
+    if rc != 0:
+        raise WLError(name, rc, lib().wl_last_error().decode())
+    return rc
+
+
+def last_error() -> str:
+    return lib().wl_last_error().decode()

@@ -1,0 +1,139 @@
+"""Build libwl_amd.so (HIP, gfx950) and the prerun/mainrun executables in-tree.
+
+Everything is compiled with hipcc for gfx950 with -ffp-contract=off, so the
+host and device halves of the shared FP64 code evaluate the same operation
+sequences.  Outputs go to ``qec_pf_wang_landau_amd/_lib/``; objects to
+``qec_pf_wang_landau_amd/_build/``.
+"""
+
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+OUT = PKG / "_lib"
+OBJ = PKG / "_build"
+INCLUDE = ROOT / "include"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("WL_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["wl_plain.hip", "wl_fused.hip"]
+HOST_SOURCES = ["wl_host.cpp"]
+EXECUTABLES: dict[str, str] = {}
+
+COMMON = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    f"-I{INCLUDE}",
+    f"-I{CSRC}",
+]
+DEVICE_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+]
+
+HDF5_PREFIXES = [os.environ.get("WL_HDF5_PREFIX", ""), "/opt/conda", "/usr", "/usr/lib/x86_64-linux-gnu/hdf5/serial"]
+
+
+def hdf5_flags() -> tuple[list[str], list[str]] | None:
+    """Compile and link flags for the HDF5 C library, or None if absent."""
+    for prefix in HDF5_PREFIXES:
+        if not prefix:
+            continue
+        inc = Path(prefix) / "include" / "hdf5.h"
+        for libdir in (Path(prefix) / "lib", Path(prefix) / "lib64", Path(prefix)):
+            so = libdir / "libhdf5.so"
+            if inc.exists() and so.exists():
+                # Link through a private directory that holds only libhdf5, so
+                # the executables' run path cannot pull in that prefix's
+                # other libraries (an older libstdc++ in a conda prefix).
+                priv = OUT / "hdf5"
+                priv.mkdir(parents=True, exist_ok=True)
+                real = so.resolve()
+                for name in ("libhdf5.so", real.name, _soname(real)):
+                    link = priv / name
+                    if name and not link.exists():
+                        link.symlink_to(real)
+                return ([f"-I{inc.parent}", "-DWL_WITH_HDF5=1"],
+                        [f"-L{priv}", "-lhdf5", "-Wl,-rpath,$ORIGIN/../hdf5"])
+    return None
+
+
+def _soname(path: Path) -> str:
+    out = subprocess.run(["readelf", "-d", str(path)], capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        if "SONAME" in line and "[" in line:
+            return line.split("[", 1)[1].rstrip("]")
+    return ""
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        msg = "\n".join(line for line in proc.stderr.splitlines() if "packed-fp32-ops" not in line)
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}\n{msg}")
+
+
+def _compile(src: Path, obj: Path, device: bool, extra: list[str]) -> Path:
+    cmd = [HIPCC, *COMMON, *extra, "-c", str(src), "-o", str(obj)]
+    if device:
+        cmd[1:1] = DEVICE_FLAGS
+    else:
+        cmd[1:1] = ["-x", "c++"]
+    _run(cmd)
+    return obj
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile everything; returns the path of libwl_amd.so."""
+    OUT.mkdir(exist_ok=True)
+    OBJ.mkdir(exist_ok=True)
+    h5 = hdf5_flags()
+    h5_c, h5_l = h5 if h5 else ([], [])
+    jobs = []
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        for name in HIP_SOURCES:
+            jobs.append(ex.submit(_compile, CSRC / name, OBJ / (name + ".o"), True, []))
+        for name in HOST_SOURCES:
+            jobs.append(ex.submit(_compile, CSRC / name, OBJ / (name + ".o"), False, h5_c))
+        exe_objs = {}
+        for exe, src in EXECUTABLES.items():
+            exe_objs[exe] = ex.submit(_compile, CSRC / src, OBJ / (src + ".o"), False, h5_c)
+        objs = [j.result() for j in jobs]
+        exe_objs = {k: v.result() for k, v in exe_objs.items()}
+    lib = OUT / "libwl_amd.so"
+    tmp = OUT / "libwl_amd.so.tmp"
+    _run([HIPCC, *DEVICE_FLAGS, "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, lib)
+    bindir = OUT / "bin"
+    bindir.mkdir(exist_ok=True)
+    for exe, obj in exe_objs.items():
+        _run([HIPCC, *DEVICE_FLAGS, "-o", str(bindir / exe), str(obj), f"-L{OUT}", "-lwl_amd",
+              "-Wl,-rpath,$ORIGIN/..", *h5_l])
+    if verbose:
+        print(f"built {lib} (hdf5: {'yes' if h5 else 'no'})")
+    return lib
+
+
+def clean() -> None:
+    shutil.rmtree(OUT, ignore_errors=True)
+    shutil.rmtree(OBJ, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,667 @@
+// Batched persistent form of the replica-exchange Wang-Landau mainrun.
+//
+// One workgroup runs one whole (disorder sample, error class) task: every
+// interval, every walker, every round of sweep -> flatness -> averaging ->
+// reset -> finished check -> replica exchange (src/mainrun.cu:243-383), with
+// no host round trip.  Workgroups pull tasks from a device queue until it is
+// empty, so tasks of different sizes share one launch.
+//
+// LDS layout of a task (all counts from the task descriptor):
+//   Jh[X], Jv[X]          coupling signs as bits (1 = antiferromagnetic), one
+//                         u32 row per lattice row (Y <= 32)
+//   mask[R]               spectrum flags over [E_min, E_max]
+//   per interval          start, end, first reachable energy e0, compact bin
+//                         count nb, bin base, cond, flat vote, factor f
+//   G[bins][W]            ln g per walker over the interval's reachable
+//                         energies only (E = e0 + stride*b), walker-minor
+//   Hc[bins][W]           histogram, u32, same indexing
+//   lat[M][X]             lattices as bits (1 = spin -1), one u32 per row
+//   EL[M], slot[M], idx[M] energy, lattice slot and replica index per walker
+// Lattices never move: replica exchange swaps slots and energies, as the
+// reference swaps lattice offsets (src/wl.cu:880-885).
+//
+// Reachable energies of a periodic lattice differ by multiples of 4 (stride 2
+// for open boundaries), so only every stride-th energy has a bin; energies
+// outside the spectrum never receive updates, so the compaction is exact.
+//
+// Semantics are those of the plain kernels in wl_plain.hip, step for step:
+// same Philox streams and draw counts, same operation order in every FP64
+// update, so the final state is bit-identical to wl_mainrun_loop.
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+#include <vector>
+
+#include "wl_amd.h"
+#include "wl_common.h"
+
+namespace wlamd {
+namespace {
+
+constexpr int kMaxBlock = 1024;
+
+struct lds_plan {
+  int X, Y, N, I, W, M, R, stride_shift, total_bins;
+  // byte offsets
+  int o_jh, o_jv, o_mask, o_sk, o_ek, o_e0, o_nb, o_gb, o_cond, o_vote, o_f, o_G, o_H, o_lat, o_E, o_slot, o_idx,
+    o_misc, bytes;
+};
+
+// Bin count of interval k for reachable energies e_min + stride*m.
+__host__ __device__ inline void interval_bins(int sk, int ek, int emin, int shift, int& e0, int& nb)
+{
+  const int stride = 1 << shift;
+  const int off = sk - emin;  // >= 0
+  e0 = emin + ((off + stride - 1) >> shift) * stride;
+  nb = (ek >= e0) ? ((ek - e0) >> shift) + 1 : 0;
+}
+
+__host__ __device__ inline int align8(int x)
+{
+  return (x + 7) & ~7;
+}
+
+// Fills the byte offsets; total_bins must already be set.
+__host__ __device__ inline void plan_offsets(lds_plan& p)
+{
+  int o = 0;
+  p.o_jh = o;
+  o += 4 * p.X;
+  p.o_jv = o;
+  o += 4 * p.X;
+  p.o_mask = o;
+  o = align8(o + p.R);
+  p.o_sk = o;
+  o += 4 * p.I;
+  p.o_ek = o;
+  o += 4 * p.I;
+  p.o_e0 = o;
+  o += 4 * p.I;
+  p.o_nb = o;
+  o += 4 * p.I;
+  p.o_gb = o;
+  o += 4 * p.I;
+  p.o_cond = o;
+  o += 4 * p.I;
+  p.o_vote = o;
+  o = align8(o + 4 * p.I);
+  p.o_f = o;
+  o += 8 * p.I;
+  p.o_G = o;
+  o += 8 * p.total_bins * p.W;
+  p.o_H = o;
+  o += 4 * p.total_bins * p.W;
+  p.o_lat = o;
+  o += 4 * p.M * p.X;
+  p.o_E = o;
+  o += 4 * p.M;
+  p.o_slot = o;
+  o += 4 * p.M;
+  p.o_idx = o;
+  o += 4 * p.M;
+  p.o_misc = o;
+  o += 16;
+  p.bytes = align8(o);
+}
+
+int shift_of(int stride)
+{
+  return stride == 4 ? 2 : (stride == 2 ? 1 : 0);
+}
+
+// Host-side plan from the task descriptor and its interval table.
+bool host_plan(const wl_task& t, const int* h_start, const int* h_end, lds_plan& p)
+{
+  p.X = t.X;
+  p.Y = t.Y;
+  p.N = t.X * t.Y;
+  p.I = t.I;
+  p.W = t.W;
+  p.M = t.I * t.W;
+  if (t.X < 2 || t.Y < 2 || t.Y > 32 || t.I < 1 || t.W < 1 || p.M > kMaxBlock) {
+    return false;
+  }
+  const int emin = h_start[0];
+  const int emax = h_end[t.I - 1];
+  p.R = emax - emin + 1;
+  p.stride_shift = shift_of(t.energy_stride);
+  p.total_bins = 0;
+  for (int k = 0; k < t.I; ++k) {
+    int e0, nb;
+    interval_bins(h_start[k], h_end[k], emin, p.stride_shift, e0, nb);
+    p.total_bins += nb;
+  }
+  plan_offsets(p);
+  return p.bytes <= 160 * 1024;
+}
+
+struct task_state {
+  int8_t* lattice;
+  const int8_t* J;
+  uint64_t* H;
+  double* logG;
+  double* factor;
+  int* E;
+  uint64_t* off_iter;
+  int* off_lattice;
+  int* idx;
+  int8_t* cond;
+  int* cond_sample;
+  const int* start;
+  const int* end;
+  const int8_t* spectrum;
+  int64_t* counter;
+  int64_t* rounds;
+  int* status;
+};
+
+template<typename T>
+__device__ __forceinline__ T* lds_at(unsigned char* base, int off)
+{
+  return reinterpret_cast<T*>(base + off);
+}
+
+// Unsigned division by a small constant through a 32-bit reciprocal:
+// exact for n * d < 2^26 (sites of a lattice with at most 1024 spins).
+__device__ __forceinline__ int div_small(int n, uint32_t recip)
+{
+  return static_cast<int>(__umulhi(static_cast<uint32_t>(n), recip));
+}
+
+__device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned char* lds, int64_t max_rounds)
+{
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  lds_plan p;
+  p.X = t.X;
+  p.Y = t.Y;
+  p.N = t.X * t.Y;
+  p.I = t.I;
+  p.W = t.W;
+  p.M = t.I * t.W;
+  p.stride_shift = t.energy_stride == 4 ? 2 : (t.energy_stride == 2 ? 1 : 0);
+  const int* g_start = st.start + t.interval_off;
+  const int* g_end = st.end + t.interval_off;
+  const int emin = g_start[0];
+  const int emax = g_end[t.I - 1];
+  p.R = emax - emin + 1;
+  {
+    int tb = 0;
+    for (int k = 0; k < t.I; ++k) {
+      int e0, nb;
+      interval_bins(g_start[k], g_end[k], emin, p.stride_shift, e0, nb);
+      tb += nb;
+    }
+    p.total_bins = tb;
+  }
+  plan_offsets(p);
+  const int X = p.X, Y = p.Y, N = p.N, I = p.I, W = p.W, M = p.M;
+  const int shift = p.stride_shift;
+  const int stride = 1 << shift;
+
+  uint32_t* Jh = lds_at<uint32_t>(lds, p.o_jh);
+  uint32_t* Jv = lds_at<uint32_t>(lds, p.o_jv);
+  int8_t* mask = lds_at<int8_t>(lds, p.o_mask);
+  int* Lsk = lds_at<int>(lds, p.o_sk);
+  int* Lek = lds_at<int>(lds, p.o_ek);
+  int* Le0 = lds_at<int>(lds, p.o_e0);
+  int* Lnb = lds_at<int>(lds, p.o_nb);
+  int* Lgb = lds_at<int>(lds, p.o_gb);
+  int* Lcond = lds_at<int>(lds, p.o_cond);
+  int* Lvote = lds_at<int>(lds, p.o_vote);
+  double* Lf = lds_at<double>(lds, p.o_f);
+  double* G = lds_at<double>(lds, p.o_G);
+  uint32_t* Hc = lds_at<uint32_t>(lds, p.o_H);
+  uint32_t* lat = lds_at<uint32_t>(lds, p.o_lat);
+  int* EL = lds_at<int>(lds, p.o_E);
+  int* SL = lds_at<int>(lds, p.o_slot);
+  int* IL = lds_at<int>(lds, p.o_idx);
+  int* misc = lds_at<int>(lds, p.o_misc);  // [0] new-energy flag, [1] finished, [2] overflow
+
+  const int8_t* Jg = st.J + t.couplings_off;
+  const int64_t wbase = t.walker_off;
+  const int64_t hbase = t.hist_off;
+  const int len0 = g_end[0] - g_start[0] + 1;
+
+  // ---- load ----------------------------------------------------------
+  for (int r = tid; r < X; r += nthr) {
+    uint32_t h = 0, v = 0;
+    for (int c = 0; c < Y; ++c) {
+      h |= static_cast<uint32_t>(Jg[r * Y + c] < 0) << c;
+      v |= static_cast<uint32_t>(Jg[N + r * Y + c] < 0) << c;
+    }
+    Jh[r] = h;
+    Jv[r] = v;
+  }
+  for (int e = tid; e < p.R; e += nthr) {
+    mask[e] = st.spectrum[t.spectrum_off + e];
+  }
+  if (tid == 0) {
+    int gb = 0;
+    for (int k = 0; k < I; ++k) {
+      int e0, nb;
+      interval_bins(g_start[k], g_end[k], emin, shift, e0, nb);
+      Lsk[k] = g_start[k];
+      Lek[k] = g_end[k];
+      Le0[k] = e0;
+      Lnb[k] = nb;
+      Lgb[k] = gb;
+      gb += nb * W;
+      Lcond[k] = st.cond[t.interval_off + k];
+      Lf[k] = st.factor[wbase + static_cast<int64_t>(k) * W];
+    }
+    misc[0] = 0;
+    misc[1] = 0;
+    misc[2] = 0;
+  }
+  for (int l = tid; l < M; l += nthr) {
+    const int8_t* src = st.lattice + t.lattice_off + static_cast<int64_t>(l) * N;
+    for (int r = 0; r < X; ++r) {
+      uint32_t b = 0;
+      for (int c = 0; c < Y; ++c) {
+        b |= static_cast<uint32_t>(src[r * Y + c] < 0) << c;
+      }
+      lat[l * X + r] = b;
+    }
+    EL[l] = st.E[wbase + l];
+    SL[l] = static_cast<int>((st.off_lattice[wbase + l] - t.lattice_off) / N);
+    IL[l] = st.idx[wbase + l];
+  }
+  __syncthreads();
+  // ln g and H: one (walker, bin) pair per lane-iteration.
+  for (int l = tid; l < M; l += nthr) {
+    const int k = l / W, w = l - (l / W) * W;
+    const int lenk = Lek[k] - Lsk[k] + 1;
+    const int64_t oh = hbase + static_cast<int64_t>(k) * W * len0 + static_cast<int64_t>(w) * lenk;
+    const int nb = Lnb[k], gb = Lgb[k], e0 = Le0[k], sk = Lsk[k];
+    for (int b = 0; b < nb; ++b) {
+      const int e = e0 + b * stride - sk;
+      G[gb + b * W + w] = st.logG[oh + e];
+      const uint64_t h = st.H[oh + e];
+      Hc[gb + b * W + w] = static_cast<uint32_t>(h);
+      if (h > 0xffffffffull) {
+        misc[2] = 1;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- per-lane walker registers ---------------------------------------
+  const bool walker = tid < M;
+  const int k = walker ? tid / W : 0;
+  const int w = walker ? tid - k * W : 0;
+  const uint32_t recipY = static_cast<uint32_t>((0x100000000ull + Y - 1) / Y);
+  const int per_sample = M;
+  stream_reader rng;
+  rng.init(t.rng_key, kTagWalk, static_cast<uint32_t>(tid % per_sample), walker ? st.off_iter[wbase + tid] : 0);
+  const double ebeta = wl_exp(t.beta);
+  const bool compat = (t.flags & 1) != 0;
+  int64_t counter = st.counter[q];
+  int64_t rounds_done = 0;
+  int status = 0;
+  const int8_t* maskE = mask - emin;  // maskE[E]
+
+  while (rounds_done < max_rounds) {
+    // ---- sweep: num_iterations WL attempts per walker -----------------
+    if (walker) {
+      const int sk = Lsk[k], ek = Lek[k], e0 = Le0[k], gb = Lgb[k];
+      double* Gw = G + gb + w;
+      uint32_t* Hw = Hc + gb + w;
+      const bool converged = Lcond[k] != 0;
+      const double lnf = wl_log(Lf[k]);
+      uint32_t* L = lat + SL[tid] * X;
+      int e_cur = EL[tid];
+      int b_cur = (e_cur - e0) >> shift;
+      double g_cur = Gw[b_cur * W];
+      for (int it = 0; it < t.num_iterations; ++it) {
+        const int site = pick_index(rng.next(), N);
+        const int i = div_small(site, recipY);
+        const int j = site - i * Y;
+        const int ipp = (i + 1 < X) ? i + 1 : 0;
+        const int inn = (i - 1 >= 0) ? i - 1 : X - 1;
+        const int jpp = (j + 1 < Y) ? j + 1 : 0;
+        const int jnn = (j - 1 >= 0) ? j - 1 : Y - 1;
+        const uint32_t rc = L[i], ru = L[inn], rd = L[ipp];
+        const uint32_t jh = Jh[i], jvu = Jv[inn], jvd = Jv[i];
+        const uint32_t sbit = (rc >> j) & 1u;
+        // mismatch (s_nb * J == -1) per neighbour
+        const uint32_t mu = ((ru >> j) ^ (jvu >> j)) & 1u;
+        const uint32_t md = ((rd >> j) ^ (jvd >> j)) & 1u;
+        const uint32_t ml = ((rc >> jnn) ^ (jh >> jnn)) & 1u;
+        const uint32_t mr = ((rc >> jpp) ^ (jh >> j)) & 1u;
+        int sum;
+        if (t.boundary == 1) {
+          const int cu = i > 0, cd = i < X - 1, cl = j > 0, cr = j < Y - 1;
+          sum = cu * (1 - 2 * static_cast<int>(mu)) + cd * (1 - 2 * static_cast<int>(md)) +
+                cl * (1 - 2 * static_cast<int>(ml)) + cr * (1 - 2 * static_cast<int>(mr));
+        } else {
+          sum = 4 - 2 * static_cast<int>(mu + md + ml + mr);
+        }
+        const int en = e_cur + (sbit ? -2 * sum : 2 * sum);
+        if (en < emin || en > emax || maskE[en] != 1) {
+          misc[0] = 1;
+          break;
+        }
+        const bool in_range = en >= sk && en <= ek;
+        if (!converged) {
+          if (in_range) {
+            const int b_new = (en - e0) >> shift;
+            const double g_new = Gw[b_new * W];
+            const double pacc = accept_prob(g_cur - g_new);
+            if (rng.next() < pacc) {
+              L[i] = rc ^ (1u << j);
+              e_cur = en;
+              b_cur = b_new;
+              g_cur = g_new;
+            }
+          }
+          g_cur = g_cur + lnf;
+          Gw[b_cur * W] = g_cur;
+          Hw[b_cur * W] += 1u;
+        } else if (in_range) {
+          const int b_new = (en - e0) >> shift;
+          const double g_new = Gw[b_new * W];
+          const double pacc = accept_prob(g_cur - g_new);
+          if (rng.next() < pacc) {
+            L[i] = rc ^ (1u << j);
+            e_cur = en;
+            b_cur = b_new;
+            g_cur = g_new;
+          }
+        }
+      }
+      EL[tid] = e_cur;
+    }
+    for (int kk = tid; kk < I; kk += nthr) {
+      Lvote[kk] = 0;
+    }
+    __syncthreads();
+    if (misc[0] != 0) {
+      status = 2;
+      rounds_done += 1;
+      break;
+    }
+    // ---- flatness (src/wl.cu:892-967) --------------------------------
+    if (walker && Lcond[k] == 0) {
+      const int e0 = Le0[k], nb = Lnb[k];
+      const uint32_t* Hw = Hc + Lgb[k] + w;
+      uint32_t mn = 0xffffffffu;
+      uint64_t sum = 0;
+      int cnt = 0;
+      for (int b = 0; b < nb; ++b) {
+        if (maskE[e0 + b * stride] == 1) {
+          const uint32_t h = Hw[b * W];
+          mn = (h < mn) ? h : mn;
+          sum += h;
+          cnt += 1;
+        }
+      }
+      if (cnt > 0) {
+        const double mean = static_cast<double>(sum) / cnt;
+        if (static_cast<double>(mn) >= t.alpha * mean) {
+          atomicAdd(&Lvote[k], 1);
+        }
+      }
+    }
+    __syncthreads();
+    if (walker && Lcond[k] == 0 && Lvote[k] == W) {
+      uint32_t* Hw = Hc + Lgb[k] + w;
+      for (int b = 0; b < Lnb[k]; ++b) {
+        Hw[b * W] = 0u;
+      }
+    }
+    __syncthreads();
+    for (int kk = tid; kk < I; kk += nthr) {
+      if (Lcond[kk] == 0 && Lvote[kk] == W) {
+        Lcond[kk] = 1;
+        Lf[kk] = wl_sqrt(Lf[kk]);
+      }
+    }
+    __syncthreads();
+    // ---- averaging + redistribution (src/wl.cu:710-783, 969-1031) -------
+    for (int pr = tid; pr < p.total_bins; pr += nthr) {
+      // locate the interval of compact bin pr
+      int kk = 0;
+      while (kk + 1 < I && Lgb[kk + 1] <= pr * W) {
+        ++kk;
+      }
+      if (Lcond[kk] != 1) {
+        continue;
+      }
+      const int b = pr - Lgb[kk] / W;
+      double* row = G + Lgb[kk] + b * W;
+      double acc = 0.0;
+      if (maskE[Le0[kk] + b * stride] == 1) {
+        const double dW = static_cast<double>(W);
+        for (int ww = 0; ww < W; ++ww) {
+          acc += row[ww] / dW;
+        }
+      }
+      for (int ww = 0; ww < W; ++ww) {
+        row[ww] = acc;
+      }
+    }
+    __syncthreads();
+    // ---- reset + finished (src/wl.cu:596-613, 662-708) -------------------
+    if (tid == 0) {
+      int sum = 0;
+      for (int kk = 0; kk < I; ++kk) {
+        if (Lcond[kk] == 1 && Lf[kk] > ebeta) {
+          Lcond[kk] = 0;
+        }
+        sum += Lcond[kk];
+      }
+      misc[1] = (sum == I) ? 1 : 0;
+    }
+    __syncthreads();
+    rounds_done += 1;
+    if (misc[1] != 0) {
+      // The finished round skips the exchange but still advances the
+      // counter, as the reference loop does for a finished sample.
+      if (!compat) {
+        counter += 1;
+      }
+      status = 1;
+      break;
+    }
+    // ---- replica exchange (src/wl.cu:785-890) ------------------------
+    if (counter % t.exchange_every == 0 && I > 1) {
+      for (int pass = 0; pass < 2; ++pass) {
+        const int even = (pass == 0);
+        const int par = walker ? ((compat ? (t.parity_base + k) : k) & 1) : 1;
+        const bool part = walker && k < I - 1 && ((even && par == 0) || (!even && par != 0));
+        if (part && w == 0) {
+          int* seg = IL + k * W;
+          for (int ii = W - 1; ii > 0; --ii) {
+            const int r = pick_index(rng.next(), ii + 1);
+            const int tmp = seg[ii];
+            seg[ii] = seg[r];
+            seg[r] = tmp;
+          }
+        }
+        __syncthreads();
+        if (part) {
+          const int ps = IL[tid];
+          const int pt = (k + 1) * W + ps;
+          const int Et = EL[tid];
+          const int Es = EL[pt];
+          if (Et <= Lek[k + 1] && Et >= Lsk[k + 1] && Es <= Lek[k] && Es >= Lsk[k]) {
+            const double* Gi = G + Lgb[k] + w;
+            const double* Gj = G + Lgb[k + 1] + ps;
+            const double giEX = Gi[((Et - Le0[k]) >> shift) * W];
+            const double giEY = Gi[((Es - Le0[k]) >> shift) * W];
+            const double gjEY = Gj[((Es - Le0[k + 1]) >> shift) * W];
+            const double gjEX = Gj[((Et - Le0[k + 1]) >> shift) * W];
+            const double pacc = accept_prob((giEX - giEY) + (gjEY - gjEX));
+            if (rng.next() < pacc) {
+              const int sl = SL[tid];
+              SL[tid] = SL[pt];
+              SL[pt] = sl;
+              EL[tid] = Es;
+              EL[pt] = Et;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (!compat) {
+      counter += 1;
+    }
+  }
+
+  // ---- write back --------------------------------------------------------
+  __syncthreads();
+  for (int l = tid; l < M; l += nthr) {
+    int8_t* dst = st.lattice + t.lattice_off + static_cast<int64_t>(l) * N;
+    for (int r = 0; r < X; ++r) {
+      const uint32_t b = lat[l * X + r];
+      for (int c = 0; c < Y; ++c) {
+        dst[r * Y + c] = ((b >> c) & 1u) ? -1 : 1;
+      }
+    }
+    const int kk = l / W;
+    st.E[wbase + l] = EL[l];
+    st.off_lattice[wbase + l] = static_cast<int>(t.lattice_off + static_cast<int64_t>(SL[l]) * N);
+    st.idx[wbase + l] = IL[l];
+    st.factor[wbase + l] = Lf[kk];
+  }
+  if (walker) {
+    st.off_iter[wbase + tid] = rng.draw;
+  }
+  for (int l = tid; l < M; l += nthr) {
+    const int kk = l / W, ww = l - (l / W) * W;
+    const int lenk = Lek[kk] - Lsk[kk] + 1;
+    const int64_t oh = hbase + static_cast<int64_t>(kk) * W * len0 + static_cast<int64_t>(ww) * lenk;
+    for (int b = 0; b < Lnb[kk]; ++b) {
+      const int e = Le0[kk] + b * stride - Lsk[kk];
+      st.logG[oh + e] = G[Lgb[kk] + b * W + ww];
+      st.H[oh + e] = Hc[Lgb[kk] + b * W + ww];
+    }
+  }
+  if (tid == 0) {
+    int sum = 0;
+    for (int kk = 0; kk < I; ++kk) {
+      st.cond[t.interval_off + kk] = static_cast<int8_t>(Lcond[kk]);
+      sum += Lcond[kk];
+    }
+    st.cond_sample[q] = (sum == I) ? -1 : sum;
+    st.counter[q] = counter;
+    st.rounds[q] += rounds_done;
+    st.status[q] = (misc[2] != 0 && status == 0) ? 4 : status;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kMaxBlock) k_fused(const wl_task* tasks, int num_tasks, int* queue, task_state st,
+                                                     int64_t max_rounds)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ int s_task;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_task = atomicAdd(queue, 1);
+    }
+    __syncthreads();
+    const int q = s_task;
+    __syncthreads();
+    if (q >= num_tasks) {
+      break;
+    }
+    const wl_task t = tasks[q];
+    if (st.status[q] != 0 || st.cond_sample[q] == -1) {
+      continue;
+    }
+    if (t.I * t.W > static_cast<int>(blockDim.x)) {
+      if (threadIdx.x == 0) {
+        st.status[q] = 3;
+      }
+      continue;
+    }
+    run_task(t, st, q, lds, max_rounds);
+  }
+}
+
+__global__ void k_zero_int(int* p)
+{
+  *p = 0;
+}
+
+}  // namespace
+}  // namespace wlamd
+
+using namespace wlamd;
+
+extern "C" {
+
+size_t wl_fused_task_lds_bytes(const wl_task* task, const int* h_start, const int* h_end)
+{
+  if (task == nullptr || h_start == nullptr || h_end == nullptr) {
+    return 0;
+  }
+  lds_plan p{};
+  return host_plan(*task, h_start, h_end, p) ? static_cast<size_t>(p.bytes) : 0;
+}
+
+int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, int8_t* d_lattice,
+                 const int8_t* d_interactions, uint64_t* d_H, double* d_logG, double* d_factor, int* d_energy,
+                 uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx, int8_t* d_cond, int* d_cond_sample,
+                 const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
+                 int64_t* d_rounds, int* d_status, int64_t max_rounds, void* stream)
+{
+  return guarded([&] {
+    require(num_tasks >= 0 && h_tasks != nullptr, "fused_run: bad task table");
+    if (num_tasks == 0) {
+      return;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // Size the launch from the largest task: block = walkers rounded to a
+    // wave, LDS = the largest plan.
+    int max_block = 64;
+    size_t max_lds = 0;
+    int max_intervals_total = 0;
+    for (int q = 0; q < num_tasks; ++q) {
+      max_intervals_total = std::max(max_intervals_total, h_tasks[q].interval_off + h_tasks[q].I);
+    }
+    std::vector<int> h_start(max_intervals_total), h_end(max_intervals_total);
+    WL_HIP(hipMemcpyAsync(h_start.data(), d_start, sizeof(int) * max_intervals_total, hipMemcpyDeviceToHost, st));
+    WL_HIP(hipMemcpyAsync(h_end.data(), d_end, sizeof(int) * max_intervals_total, hipMemcpyDeviceToHost, st));
+    WL_HIP(hipStreamSynchronize(st));
+    for (int q = 0; q < num_tasks; ++q) {
+      const wl_task& t = h_tasks[q];
+      require(t.exchange_every > 0 && t.num_iterations >= 0, "fused_run: bad task parameters");
+      require(t.energy_stride == 1 || t.energy_stride == 2 || t.energy_stride == 4,
+              "fused_run: energy_stride must be 1, 2 or 4");
+      lds_plan p{};
+      if (!host_plan(t, h_start.data() + t.interval_off, h_end.data() + t.interval_off, p)) {
+        throw std::invalid_argument("fused_run: task " + std::to_string(q) + " does not fit one workgroup");
+      }
+      max_block = std::max(max_block, round_up64(p.M));
+      max_lds = std::max(max_lds, static_cast<size_t>(p.bytes));
+    }
+    int dev = 0;
+    WL_HIP(hipGetDevice(&dev));
+    int num_cu = 0;
+    WL_HIP(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    int per_cu = 0;
+    WL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_fused), max_block,
+                                                         max_lds));
+    per_cu = std::max(per_cu, 1);
+    const int grid = std::min(num_tasks, num_cu * per_cu);
+    int* d_queue = nullptr;
+    WL_HIP(hipMalloc(&d_queue, sizeof(int)));
+    task_state s{d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy, d_offset_iter, d_offset_lattice,
+                 d_idx, d_cond, d_cond_sample, d_start, d_end, d_spectrum, d_counter, d_rounds, d_status};
+    k_zero_int<<<1, 1, 0, st>>>(d_queue);
+    k_fused<<<grid, max_block, max_lds, st>>>(d_tasks, num_tasks, d_queue, s, max_rounds);
+    const hipError_t launch = hipGetLastError();
+    const hipError_t sync = hipStreamSynchronize(st);
+    (void)hipFree(d_queue);
+    hip_check(launch, __FILE__, __LINE__);
+    hip_check(sync, __FILE__, __LINE__);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+// Host-side domain logic around the kernels: the interval partition
+// (src/interval.cpp), spectrum extraction and seed-lattice selection from a
+// prerun (src/file_io.cpp:558-659), stitching (src/result_handling.cpp) and
+// log Z for the logical-class decision.  C entry points in include/wl_host.h.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "wl_host.h"
+#include "wl_math.h"
+#include "wl_stitch.h"
+
+namespace wlamd {
+void set_last_error(const std::string& msg);
+}
+
+namespace {
+
+template<typename F>
+int guarded_host(F&& fn)
+{
+  try {
+    fn();
+    return 0;
+  } catch (const std::invalid_argument& e) {
+    wlamd::set_last_error(e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    wlamd::set_last_error(e.what());
+    return 2;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// src/interval.cpp:5-41.  The float arithmetic of the reference is kept:
+// len = trunc(R / (1 + o (I - 1))) and step = trunc(o len) in float.
+int wl_interval_results(int e_min, int e_max, unsigned int num_intervals, int num_walkers, float overlap, int* h_start,
+                        int* h_end, int64_t* len_histogram_over_all_walkers, int* len_interval)
+{
+  return guarded_host([&] {
+    if (overlap > 1 || overlap < 0) {
+      throw std::invalid_argument("Overlap is out of bounds.");
+    }
+    if (num_intervals == 0) {
+      throw std::invalid_argument("The interval arguments create empty intervals.");
+    }
+    const int e_range = e_max - e_min + 1;
+    const float denom = 1.0F + overlap * static_cast<float>(num_intervals - 1);
+    const int len = static_cast<int>(static_cast<float>(e_range) / denom);
+    const int step = static_cast<int>(overlap * static_cast<float>(len));
+    int64_t total = 0;
+    int s = e_min;
+    for (unsigned int i = 0; i < num_intervals; ++i) {
+      h_start[i] = s;
+      h_end[i] = (i + 1 < num_intervals) ? s + len - 1 : e_max;
+      total += static_cast<int64_t>(num_walkers) * (h_end[i] - h_start[i] + 1);
+      if (h_start[i] >= h_end[i]) {
+        throw std::invalid_argument("The interval arguments create empty intervals.");
+      }
+      s += step;
+    }
+    *len_histogram_over_all_walkers = total;
+    *len_interval = len;
+  });
+}
+
+// src/file_io.cpp:558-616 for one sample: E_min/E_max are the first and last
+// visited energies of the prerun histogram over [-2N, 2N]; the mask is the
+// histogram's 0/1 flags between them.
+int wl_energy_spectrum(const uint8_t* hist, int len_hist, int X, int Y, int* E_min, int* E_max, int8_t* mask,
+                       int* len_mask)
+{
+  return guarded_host([&] {
+    if (len_hist != 4 * X * Y + 1) {
+      throw std::invalid_argument("histogram length does not match 4XY+1");
+    }
+    int first = -1, last = -1;
+    for (int e = 0; e < len_hist; ++e) {
+      if (hist[e] == 1) {
+        if (first < 0) {
+          first = e;
+        }
+        last = e;
+      }
+    }
+    if (first < 0) {
+      throw std::runtime_error("Error: Histogram does not contain 1.");
+    }
+    *E_min = -2 * X * Y + first;
+    *E_max = -2 * X * Y + last;
+    *len_mask = last - first + 1;
+    for (int e = first; e <= last; ++e) {
+      mask[e - first] = static_cast<int8_t>(hist[e]);
+    }
+  });
+}
+
+// Spacing of the energies present in a spectrum mask (gcd of their
+// differences; 0 for a single energy).
+int wl_energy_stride(const int8_t* mask, int len)
+{
+  int g = 0, prev = -1;
+  for (int e = 0; e < len; ++e) {
+    if (mask[e] == 1) {
+      if (prev >= 0) {
+        g = std::gcd(g, e - prev);
+      }
+      prev = e;
+    }
+  }
+  return g;
+}
+
+// src/file_io.cpp:618-659 for one sample: for every interval, the lowest-energy
+// found lattice with energy in [start, end], replicated for its W walkers.
+int wl_select_seed_lattices(const int8_t* found, const int* found_energy, int num_found, int N, const int* start,
+                            const int* end, int I, int W, int8_t* out)
+{
+  return guarded_host([&] {
+    std::map<int, int> by_energy;  // energy -> index of the found lattice
+    for (int q = 0; q < num_found; ++q) {
+      by_energy.emplace(found_energy[q], q);
+    }
+    for (int k = 0; k < I; ++k) {
+      int pick = -1;
+      for (const auto& [e, q] : by_energy) {
+        if (e >= start[k] && e <= end[k]) {
+          pick = q;
+          break;
+        }
+      }
+      if (pick < 0) {
+        throw std::runtime_error("Error: Did not find lattice configuration with suiting energy for interval [" +
+                                 std::to_string(start[k]) + ", " + std::to_string(end[k]) + "]");
+      }
+      for (int w = 0; w < W; ++w) {
+        std::memcpy(out + (static_cast<int64_t>(k) * W + w) * N, found + static_cast<int64_t>(pick) * N,
+                    static_cast<size_t>(N));
+      }
+    }
+  });
+}
+
+int wl_stitch_log_g(const double* logG_sample, const int* start, const int* end, int I, int W, int X, int Y,
+                    int* out_E, double* out_lng, int max_out, int* n_out)
+{
+  return guarded_host([&] {
+    if (I < 1 || W < 1) {
+      throw std::invalid_argument("stitch: bad shape");
+    }
+    int total = 0;
+    for (int i = 0; i < I; ++i) {
+      total += end[i] - start[i] + 1;
+    }
+    std::vector<double> vals(total);
+    std::vector<uint8_t> pres(total);
+    std::vector<int> nxt(total), kiv(I), ke(I);
+    wlamd::stitch_work wk{vals.data(), pres.data(), nxt.data(), kiv.data(), ke.data()};
+    const int n = wlamd::stitch_dense(logG_sample, start, end, I, W, X, Y, wk, out_E, out_lng, max_out);
+    if (n == -2) {
+      throw std::runtime_error("stitch: no overlapping intervals to stitch");
+    }
+    if (n == -3) {
+      throw std::runtime_error("stitch: stitching energy not found in one of the intervals");
+    }
+    if (n == -4) {
+      throw std::invalid_argument("stitch: output buffer too small");
+    }
+    *n_out = n;
+  });
+}
+
+double wl_log_partition(const int* E, const double* lng, int n, double beta)
+{
+  return wlamd::log_partition(E, lng, n, beta);
+}
+
+double wl_nishimori_beta(double p)
+{
+  return 0.5 * wlamd::wl_log((1.0 - p) / p);
+}
+
+}  // extern "C"

@@ -1,0 +1,271 @@
+// Deterministic random numbers and FP64 elementary functions shared by the
+// host and the gfx950 device code of the replica-exchange Wang-Landau engine.
+//
+// Every function here is a fixed sequence of IEEE-754 operations (add, mul,
+// fma, division, conversions, bit manipulation).  Compiled with
+// -ffp-contract=off on both sides, the host and the device produce the same
+// bits, which is what makes walker trajectories, histograms and exchange
+// decisions reproducible across CPU and GPU.
+//
+// Random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC'11).  The reference draws from cuRAND's
+// Philox4_32_10 streams keyed by (seed + sample, subsequence = walker slot,
+// offset = draw counter) (src/wl.cu:1074); here the same three coordinates
+// key a counter-based Philox block directly:
+//   key     = (seed + sample, stream tag)
+//   counter = (draw / 4 as 64 bits, walker slot, 0)
+//   uniform = word (draw % 4) of that block, mapped to (0, 1).
+//
+// exp: Cody & Waite argument reduction (Cody, Waite, "Software Manual for the
+// Elementary Functions", 1980) with the fdlibm two-part ln 2, followed by a
+// degree-13 Taylor polynomial in Horner form.  log: exponent/mantissa split
+// and the atanh series log(m) = 2 atanh((m-1)/(m+1)).  sqrt: reciprocal
+// square-root Newton iteration from a bit-level seed plus one residual
+// correction.  All are accurate to about one ulp; their purpose is bit-equal
+// host/device results, not correct rounding.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define WL_HD __host__ __device__ __forceinline__
+#else
+#define WL_HD inline
+#endif
+
+namespace wlamd {
+
+// Stream tags (second Philox key word).
+enum : uint32_t {
+  kTagWalk = 0u,      // mainrun walker slot streams (WL sweep, shuffle, exchange)
+  kTagDisorder = 1u,  // coupling signs, one draw per bond
+  kTagLattice = 2u,   // prerun initial lattices
+  kTagPrerun = 3u,    // prerun walker streams
+};
+
+WL_HD uint64_t dbl_bits(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return static_cast<uint64_t>(__double_as_longlong(x));
+#else
+  uint64_t b;
+  std::memcpy(&b, &x, sizeof b);
+  return b;
+#endif
+}
+
+WL_HD double bits_dbl(uint64_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __longlong_as_double(static_cast<long long>(b));
+#else
+  double x;
+  std::memcpy(&x, &b, sizeof x);
+  return x;
+#endif
+}
+
+WL_HD double wl_fma(double a, double b, double c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_fma(a, b, c);
+#else
+  return __builtin_fma(a, b, c);
+#endif
+}
+
+struct philox4 {
+  uint32_t v[4];
+};
+
+// Philox4x32 with 10 rounds.  Round: (hi0,lo0)=M0*c0, (hi1,lo1)=M1*c2,
+// c <- (hi1^c1^k0, lo1, hi0^c3^k1, lo0); the key is bumped by the Weyl
+// constants between rounds.
+WL_HD philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
+{
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = static_cast<uint64_t>(M0) * c0;
+    const uint64_t p1 = static_cast<uint64_t>(M1) * c2;
+    const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+    const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
+    const uint32_t n0 = hi1 ^ c1 ^ k0;
+    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+  return philox4{{c0, c1, c2, c3}};
+}
+
+// Map a 32-bit word to a double in (0, 1): (x + 1/2) * 2^-32, exact.
+WL_HD double u32_to_unit(uint32_t x)
+{
+  return (static_cast<double>(x) + 0.5) * 0x1p-32;
+}
+
+WL_HD philox4 stream_block(int seed_plus_sample, uint32_t tag, uint32_t slot, uint64_t block)
+{
+  return philox4x32_10(static_cast<uint32_t>(block), static_cast<uint32_t>(block >> 32), slot, 0u,
+                       static_cast<uint32_t>(seed_plus_sample), tag);
+}
+
+// Uniform number `draw` of stream (seed + sample, tag, slot).
+WL_HD double stream_uniform(int seed_plus_sample, uint32_t tag, uint32_t slot, uint64_t draw)
+{
+  const philox4 b = stream_block(seed_plus_sample, tag, slot, draw >> 2);
+  return u32_to_unit(b.v[draw & 3u]);
+}
+
+// Sequential reader of one stream that keeps the current Philox block in
+// registers, so four consecutive draws cost one Philox evaluation.
+struct stream_reader {
+  uint32_t key0, tag, slot;
+  uint64_t draw;   // number of uniforms consumed so far (the reference's offset_iter)
+  uint64_t block;  // block index held in `cache`, ~0 if none
+  philox4 cache;
+
+  WL_HD void init(int seed_plus_sample, uint32_t tag_, uint32_t slot_, uint64_t draw_)
+  {
+    key0 = static_cast<uint32_t>(seed_plus_sample);
+    tag = tag_;
+    slot = slot_;
+    draw = draw_;
+    block = ~0ull;
+  }
+
+  WL_HD double next()
+  {
+    const uint64_t b = draw >> 2;
+    if (b != block) {
+      cache = philox4x32_10(static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32), slot, 0u, key0, tag);
+      block = b;
+    }
+    const uint32_t w = cache.v[draw & 3u];
+    ++draw;
+    return u32_to_unit(w);
+  }
+};
+
+// Random site in [0, n) exactly as the reference forms it from a uniform:
+// trunc(u * (n - 1 + 0.999999)) (src/wl.cu:66-68).
+WL_HD int pick_index(double u, int n)
+{
+  const double scale = static_cast<double>(n - 1) + 0.999999;
+  return static_cast<int>(u * scale);
+}
+
+// 2^k for -1022 <= k <= 1023 from the exponent bits.
+WL_HD double pow2i(int k)
+{
+  return bits_dbl(static_cast<uint64_t>(k + 1023) << 52);
+}
+
+// exp(x), Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, Taylor degree 13.
+WL_HD double wl_exp(double x)
+{
+  if (!(x > -745.2)) {
+    return 0.0;
+  }
+  if (x > 709.78) {
+    return bits_dbl(0x7ff0000000000000ull);
+  }
+  const double log2e = 0x1.71547652b82fep+0;
+  const double shifter = 0x1.8p52;
+  const double ln2_hi = 0x1.62e42fee00000p-1;
+  const double ln2_lo = 0x1.a39ef35793c76p-33;
+  const double t = x * log2e;
+  const double kd = (t + shifter) - shifter;  // round to nearest integer
+  const int k = static_cast<int>(kd);
+  double r = wl_fma(-kd, ln2_hi, x);
+  r = wl_fma(-kd, ln2_lo, r);
+  double p = 0x1.6124613a86d09p-33;  // 1/13!
+  p = wl_fma(p, r, 0x1.1eed8eff8d898p-29);
+  p = wl_fma(p, r, 0x1.ae64567f544e4p-26);
+  p = wl_fma(p, r, 0x1.27e4fb7789f5cp-22);
+  p = wl_fma(p, r, 0x1.71de3a556c734p-19);
+  p = wl_fma(p, r, 0x1.a01a01a01a01ap-16);
+  p = wl_fma(p, r, 0x1.a01a01a01a01ap-13);
+  p = wl_fma(p, r, 0x1.6c16c16c16c17p-10);
+  p = wl_fma(p, r, 0x1.1111111111111p-7);
+  p = wl_fma(p, r, 0x1.5555555555555p-5);
+  p = wl_fma(p, r, 0x1.5555555555555p-3);
+  p = wl_fma(p, r, 0x1.0000000000000p-1);
+  p = wl_fma(p, r, 1.0);
+  p = wl_fma(p, r, 1.0);
+  // Scale in two normal steps so subnormal results round once.
+  const int k1 = k / 2;
+  const int k2 = k - k1;
+  return (p * pow2i(k1)) * pow2i(k2);
+}
+
+// log(x) for positive normal x: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
+// log m = 2 atanh(s), s = (m-1)/(m+1), series through s^21.
+WL_HD double wl_log(double x)
+{
+  if (!(x > 0.0)) {
+    return x == 0.0 ? -bits_dbl(0x7ff0000000000000ull) : bits_dbl(0x7ff8000000000000ull);
+  }
+  const uint64_t b = dbl_bits(x);
+  int e = static_cast<int>((b >> 52) & 0x7ffu) - 1023;
+  double m = bits_dbl((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+  if (m > 0x1.6a09e667f3bcdp+0) {
+    m = m * 0.5;
+    e += 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double q = 0x1.8618618618618p-5;   // 1/21
+  q = wl_fma(q, z, 0x1.af286bca1af28p-5);  // 1/19
+  q = wl_fma(q, z, 0x1.e1e1e1e1e1e1ep-5);  // 1/17
+  q = wl_fma(q, z, 0x1.1111111111111p-4);  // 1/15
+  q = wl_fma(q, z, 0x1.3b13b13b13b14p-4);  // 1/13
+  q = wl_fma(q, z, 0x1.745d1745d1746p-4);  // 1/11
+  q = wl_fma(q, z, 0x1.c71c71c71c71cp-4);  // 1/9
+  q = wl_fma(q, z, 0x1.2492492492492p-3);  // 1/7
+  q = wl_fma(q, z, 0x1.999999999999ap-3);  // 1/5
+  q = wl_fma(q, z, 0x1.5555555555555p-2);  // 1/3
+  const double two_s = 2.0 * s;
+  const double lm = wl_fma(two_s * z, q, two_s);
+  const double ln2_hi = 0x1.62e42fee00000p-1;
+  const double ln2_lo = 0x1.a39ef35793c76p-33;
+  const double ed = static_cast<double>(e);
+  return wl_fma(ed, ln2_hi, wl_fma(ed, ln2_lo, lm));
+}
+
+// sqrt(x) for non-negative normal x.
+WL_HD double wl_sqrt(double x)
+{
+  if (!(x > 0.0)) {
+    return x == 0.0 ? 0.0 : bits_dbl(0x7ff8000000000000ull);
+  }
+  double y = bits_dbl(0x5fe6eb50c7b537a9ull - (dbl_bits(x) >> 1));  // ~1/sqrt(x)
+  const double hx = 0.5 * x;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+  for (int i = 0; i < 5; ++i) {
+    y = y * wl_fma(-hx * y, y, 1.5);
+  }
+  const double s = x * y;
+  const double r = wl_fma(-s, s, x);
+  return wl_fma(r, 0.5 * y, s);
+}
+
+// Metropolis-type acceptance probability min(1, exp(d)) as the reference
+// forms it (src/wl.cu:1113): d >= 0 gives exactly 1.
+WL_HD double accept_prob(double d)
+{
+  return d >= 0.0 ? 1.0 : wl_exp(d);
+}
+
+}  // namespace wlamd

@@ -1,0 +1,1017 @@
+// Plain method: the reference's kernel library (src/wl.cu) re-designed for
+// gfx950 and exposed through the C ABI declared in include/wl_amd.h.
+//
+// Layout is the reference's (flat int8 lattices addressed through per-walker
+// offsets, u64 histograms and f64 ln g per walker at per-walker offsets), so
+// the entry points are a drop-in for the reference's launches.  What differs
+// is inside: launch shapes are chosen here (wave64-sized blocks instead of
+// one block of W threads per interval where the work allows it), the sweep
+// keeps its walker state in registers for the whole launch and draws its
+// randoms from a counter-based Philox block cache, averaging is a fixed-order
+// per-bin sum instead of FP64 atomics, sample lookup is direct instead of an
+// O(S) scan, and the finished check needs no scratch or host sync.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wl_amd.h"
+#include "wl_common.h"
+
+namespace wlamd {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg)
+{
+  g_last_error = msg;
+}
+
+namespace {
+
+constexpr int kBlock = 64;
+
+inline hipStream_t as_stream(void* s)
+{
+  return reinterpret_cast<hipStream_t>(s);
+}
+
+// ----------------------------------------------------------- energies -----
+
+__global__ void k_calc_energy(int boundary, int* E, const int8_t* lattice, const int8_t* J, const int* off_lattice,
+                              int X, int Y, int64_t num_lattices, int walker_per_interactions)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= num_lattices) {
+    return;
+  }
+  const int N = X * Y;
+  const int8_t* s = lattice + off_lattice[t];
+  const int8_t* Js = J + (t / walker_per_interactions) * 2 * N;
+  int e = 0;
+  for (int i = 0; i < X; ++i) {
+    const int inn = (i - 1 >= 0) ? i - 1 : X - 1;
+    for (int j = 0; j < Y; ++j) {
+      const int jnn = (j - 1 >= 0) ? j - 1 : Y - 1;
+      int up = s[inn * Y + j] * Js[N + inn * Y + j];
+      int left = s[i * Y + jnn] * Js[i * Y + jnn];
+      if (boundary == 1) {
+        up = (i > 0) ? up : 0;
+        left = (j > 0) ? left : 0;
+      }
+      e += s[i * Y + j] * (up + left);
+    }
+  }
+  E[t] = -e;
+}
+
+// ------------------------------------------------------------ disorder ----
+
+__global__ void k_init_interactions(int8_t* J, int64_t lattice_size, int num_interactions, int seed, float prob)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t per = 2 * lattice_size;
+  if (t >= per * num_interactions) {
+    return;
+  }
+  const int s = static_cast<int>(t / per);
+  const uint64_t b = static_cast<uint64_t>(t % per);
+  const double u = stream_uniform(seed + s, kTagDisorder, 0u, b);
+  J[t] = (u < static_cast<double>(prob)) ? -1 : 1;
+}
+
+__global__ void k_apply_horizontal(int8_t* J, int X, int Y, int S)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Y * S) {
+    return;
+  }
+  const int64_t s = t / Y;
+  const int col = t % Y;
+  J[s * 2 * X * Y + X * Y + col] *= -1;
+}
+
+__global__ void k_apply_vertical(int8_t* J, int X, int Y, int S)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= X * S) {
+    return;
+  }
+  const int64_t s = t / X;
+  const int row = t % X;
+  J[s * 2 * X * Y + row * Y] *= -1;
+}
+
+// One thread per spin; q is recomputed per spin from the lattice's slot
+// (draw 0), which is cheaper than publishing it.
+__global__ void k_init_lattice(int8_t* lattice, float* probs, int X, int Y, int64_t num_lattices, int seed,
+                               int walker_per_interactions)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int N = X * Y;
+  if (t >= num_lattices * N) {
+    return;
+  }
+  const int64_t l = t / N;
+  const int i = static_cast<int>(t % N);
+  const int s = static_cast<int>(l / walker_per_interactions);
+  const uint32_t slot = static_cast<uint32_t>(l % walker_per_interactions);
+  const double q = stream_uniform(seed + s, kTagLattice, slot, 0);
+  if (i == 0 && probs != nullptr) {
+    probs[l] = static_cast<float>(q);
+  }
+  lattice[t] = (stream_uniform(seed + s, kTagLattice, slot, 1 + static_cast<uint64_t>(i)) < q) ? -1 : 1;
+}
+
+__global__ void k_init_offsets_lattice(int* off, int N, int64_t num)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < num) {
+    off[t] = static_cast<int>(t * N);
+  }
+}
+
+// --------------------------------------------------------------- prerun ---
+
+// One workgroup per sample, one thread per prerun walker.  The sample's
+// histogram lives in LDS as u32 for the whole launch.
+__global__ void k_prerun(int* E, int* flag_found, int8_t* lattice, int8_t* found_lattice, uint64_t* histogram,
+                         uint64_t* seed_offset, const int8_t* J, const int* off_lattice, int E_min, int E_max,
+                         int num_iterations, int X, int Y, int seed, int num_intervals, int len_interval, int Wp,
+                         int boundary)
+{
+  extern __shared__ uint32_t lds[];
+  const int N = X * Y;
+  const int len_hist = E_max - E_min + 1;
+  uint32_t* H = lds;
+  int* claim = reinterpret_cast<int*>(H + len_hist);
+  int* flag = claim + num_intervals;
+  const int s = blockIdx.x;
+  const int w = threadIdx.x;
+  uint64_t* Hg = histogram + static_cast<int64_t>(s) * len_hist;
+  for (int e = w; e < len_hist; e += blockDim.x) {
+    H[e] = static_cast<uint32_t>(Hg[e]);
+  }
+  for (int k = w; k < num_intervals; k += blockDim.x) {
+    claim[k] = INT_MAX;
+    flag[k] = flag_found[s * num_intervals + k];
+  }
+  __syncthreads();
+  const bool active = w < Wp;
+  const int64_t t = static_cast<int64_t>(s) * Wp + w;
+  const int8_t* Js = J + static_cast<int64_t>(s) * 2 * N;
+  int8_t* lat = lattice + (active ? off_lattice[t] : 0);
+  stream_reader rng;
+  rng.init(seed + s, kTagPrerun, static_cast<uint32_t>(w), active ? seed_offset[t] : 0);
+  int e_cur = active ? E[t] : 0;
+  for (int it = 0; it < num_iterations; ++it) {
+    bool acc = false;
+    int site = 0, en = 0;
+    if (active) {
+      site = pick_index(rng.next(), N);
+      en = flip_energy_i8(lat, Js, X, Y, boundary, e_cur, site / Y, site % Y);
+      const double prob =
+        wl_exp(static_cast<double>(H[e_cur - E_min]) - static_cast<double>(H[en - E_min]));
+      acc = rng.next() < prob;
+    }
+    __syncthreads();
+    int ii = -1;
+    if (active) {
+      if (acc) {
+        lat[site] = static_cast<int8_t>(-lat[site]);
+        e_cur = en;
+        atomicAdd(&H[en - E_min], 1u);
+        ii = (e_cur - E_min) / len_interval;
+        ii = (ii < num_intervals) ? ii : num_intervals - 1;
+        if (flag[ii] == 0) {
+          atomicMin(&claim[ii], w);
+        } else {
+          ii = -1;
+        }
+      } else {
+        atomicAdd(&H[e_cur - E_min], 1u);
+      }
+    }
+    __syncthreads();
+    if (ii >= 0 && claim[ii] == w) {
+      int8_t* dst = found_lattice + (static_cast<int64_t>(s) * num_intervals + ii) * N;
+      for (int q = 0; q < N; ++q) {
+        dst[q] = lat[q];
+      }
+      flag[ii] = 1;
+      claim[ii] = INT_MAX;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    E[t] = e_cur;
+    seed_offset[t] = rng.draw;
+  }
+  for (int e = w; e < len_hist; e += blockDim.x) {
+    Hg[e] = H[e];
+  }
+  for (int k = w; k < num_intervals; k += blockDim.x) {
+    flag_found[s * num_intervals + k] = flag[k];
+  }
+}
+
+// ---------------------------------------------------- small mainrun ops ---
+
+__global__ void k_check_energy_ranges(int8_t* flags, const int* E, const int* start, const int* end, int64_t total,
+                                      int W)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= total) {
+    return;
+  }
+  const int64_t k = t / W;
+  if (E[t] > end[k] || E[t] < start[k]) {
+    flags[t] = 1;
+  }
+}
+
+__global__ void k_init_indices(int* idx, int64_t total, int W)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < total) {
+    idx[t] = static_cast<int>(t % W);
+  }
+}
+
+__global__ void k_reset_d_cond(int8_t* cond, const double* factor, int total_intervals, double beta, int W)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total_intervals) {
+    return;
+  }
+  if (cond[k] == 1 && factor[static_cast<int64_t>(k) * W] > wl_exp(beta)) {
+    cond[k] = 0;
+  }
+}
+
+// One thread per walker: base of the sample from a prefix over the samples'
+// histogram lengths (computed once per sample by lane arithmetic).
+__global__ void k_init_offsets_histogram(int* off, const int* start, const int* end, const int* len_hist, int I,
+                                         int total_walker, int W)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total_walker) {
+    return;
+  }
+  const int k = t / W;
+  const int w = t % W;
+  const int s = k / I;
+  int base = 0;
+  for (int q = 0; q < s; ++q) {
+    base += len_hist[q];
+  }
+  const int len0 = end[s * I] - start[s * I] + 1;
+  const int lenk = end[k] - start[k] + 1;
+  off[t] = base + (k % I) * W * len0 + w * lenk;
+}
+
+__global__ void k_finished(int* cond_sample, const int8_t* cond, const int* off_intervals, int I, int S)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) {
+    return;
+  }
+  int sum = 0;
+  for (int k = off_intervals[s]; k < off_intervals[s + 1]; ++k) {
+    sum += cond[k];
+  }
+  cond_sample[s] = (sum == I) ? -1 : sum;
+}
+
+// Block per interval, threads over bins; walkers summed in index order.
+__global__ void k_calc_average(double* shared, const double* logG, const int* start, const int* end,
+                               const int8_t* spectrum, const int8_t* cond, const int* off_hist, const int* off_spec,
+                               const long long* off_shared, const int* cond_sample, int W, int I)
+{
+  const int k = blockIdx.x;
+  const int s = k / I;
+  if (cond_sample[s] == -1 || cond[k] != 1) {
+    return;
+  }
+  const int sk = start[k];
+  const int len = end[k] - sk + 1;
+  const int8_t* spec = spectrum + off_spec[s] + (sk - start[s * I]);
+  const double invW = static_cast<double>(W);
+  for (int e = threadIdx.x; e < len; e += blockDim.x) {
+    if (spec[e] != 1) {
+      continue;
+    }
+    double acc = shared[off_shared[k] + e];
+    for (int w = 0; w < W; ++w) {
+      acc += logG[off_hist[static_cast<int64_t>(k) * W + w] + e] / invW;
+    }
+    shared[off_shared[k] + e] = acc;
+  }
+}
+
+__global__ void k_redistribute(double* logG, const double* shared, const int* start, const int* end,
+                               const int* cond_sample, const int* off_hist, const int8_t* cond,
+                               const long long* off_shared, int I, int W)
+{
+  const int k = blockIdx.x;
+  const int s = k / I;
+  if (cond_sample[s] == -1 || cond[k] != 1) {
+    return;
+  }
+  const int len = end[k] - start[k] + 1;
+  const double* src = shared + off_shared[k];
+  for (int q = threadIdx.x; q < W * len; q += blockDim.x) {
+    const int w = q / len;
+    const int e = q - w * len;
+    logG[off_hist[static_cast<int64_t>(k) * W + w] + e] = src[e];
+  }
+}
+
+__device__ void fisher_yates_segment(int* idx, uint64_t* off_iter, int seed_plus_sample, int64_t t0, uint32_t slot,
+                                     int W)
+{
+  stream_reader rng;
+  rng.init(seed_plus_sample, kTagWalk, slot, off_iter[t0]);
+  for (int i = W - 1; i > 0; --i) {
+    const int r = pick_index(rng.next(), i + 1);
+    const int tmp = idx[t0 + i];
+    idx[t0 + i] = idx[t0 + r];
+    idx[t0 + r] = tmp;
+  }
+  off_iter[t0] = rng.draw;
+}
+
+__global__ void k_fisher_yates(int* idx, uint64_t* off_iter, int seed, int total_intervals, int W, int per_sample)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total_intervals) {
+    return;
+  }
+  const int64_t t0 = static_cast<int64_t>(k) * W;
+  fisher_yates_segment(idx, off_iter, seed + static_cast<int>(t0 / per_sample), t0,
+                       static_cast<uint32_t>(t0 % per_sample), W);
+}
+
+// Block per interval k of the requested parity; lanes are k's walkers.
+__global__ void k_replica_exchange(int* off_lattice, int* E, int* idx, uint64_t* off_iter, const int* start,
+                                   const int* end, const int* off_hist, const int* cond_sample, const double* logG,
+                                   int even, int seed, int I, int per_sample, int global_parity)
+{
+  const int k = blockIdx.x;
+  const int kl = k % I;
+  if (kl == I - 1) {
+    return;
+  }
+  const int par = global_parity ? (k & 1) : (kl & 1);
+  if ((even && par != 0) || (!even && par == 0)) {
+    return;
+  }
+  const int W = per_sample / I;
+  const int s = k / I;
+  if (cond_sample[s] == -1) {
+    return;
+  }
+  const int64_t t0 = static_cast<int64_t>(k) * W;
+  if (threadIdx.x == 0) {
+    fisher_yates_segment(idx, off_iter, seed + s, t0, static_cast<uint32_t>(t0 % per_sample), W);
+  }
+  __syncthreads();
+  const int w = threadIdx.x;
+  if (w >= W) {
+    return;
+  }
+  const int64_t t = t0 + w;
+  const int64_t sw = static_cast<int64_t>(k + 1) * W + idx[t];
+  const int Et = E[t];
+  const int Es = E[sw];
+  if (Et > end[k + 1] || Et < start[k + 1]) {
+    return;
+  }
+  if (Es > end[k] || Es < start[k]) {
+    return;
+  }
+  const double giEX = logG[off_hist[t] + Et - start[k]];
+  const double giEY = logG[off_hist[t] + Es - start[k]];
+  const double gjEY = logG[off_hist[sw] + Es - start[k + 1]];
+  const double gjEX = logG[off_hist[sw] + Et - start[k + 1]];
+  const double p = accept_prob((giEX - giEY) + (gjEY - gjEX));
+  const double u = stream_uniform(seed + s, kTagWalk, static_cast<uint32_t>(t % per_sample), off_iter[t]);
+  if (u < p) {
+    const int ol = off_lattice[t];
+    off_lattice[t] = off_lattice[sw];
+    off_lattice[sw] = ol;
+    E[t] = Es;
+    E[sw] = Et;
+  }
+  off_iter[t] += 1;
+}
+
+// Block per interval, one lane per walker: min/sum over the spectrum bins,
+// a block vote, then the flat interval's histograms are cleared and its
+// factors square-rooted.
+__global__ void k_check_histogram(uint64_t* H, double* factor, int8_t* cond, const int* off_hist, const int* end,
+                                  const int* start, const int8_t* spectrum, const int* off_spec,
+                                  const int* cond_sample, double alpha, int W, int I)
+{
+  __shared__ int finished;
+  const int k = blockIdx.x;
+  const int s = k / I;
+  if (cond[k] == 1 || cond_sample[s] == -1) {
+    return;
+  }
+  if (threadIdx.x == 0) {
+    finished = 0;
+  }
+  __syncthreads();
+  const int w = threadIdx.x;
+  const int sk = start[k];
+  const int len = end[k] - sk + 1;
+  const int8_t* spec = spectrum + off_spec[s] + (sk - start[s * I]);
+  const int64_t t = static_cast<int64_t>(k) * W + w;
+  if (w < W) {
+    const uint64_t* h = H + off_hist[t];
+    uint64_t mn = INT_MAX;
+    uint64_t sum = 0;
+    int cnt = 0;
+    for (int i = 0; i < len; ++i) {
+      if (spec[i] == 1) {
+        const uint64_t v = h[i];
+        mn = (v < mn) ? v : mn;
+        sum += v;
+        cnt += 1;
+      }
+    }
+    if (cnt > 0) {
+      const double mean = static_cast<double>(sum) / cnt;
+      if (static_cast<double>(mn) >= alpha * mean) {
+        atomicAdd(&finished, 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (finished == W && w < W) {
+    if (w == 0) {
+      cond[k] = 1;
+    }
+    uint64_t* h = H + off_hist[t];
+    for (int i = 0; i < len; ++i) {
+      h[i] = 0;
+    }
+    factor[t] = wl_sqrt(factor[t]);
+  }
+}
+
+// ------------------------------------------------------------- sweep ------
+
+// One lane per walker.  The walker's energy, RNG block and interval bounds
+// stay in registers for all num_iterations attempts; spins, couplings,
+// histogram and ln g are read from global memory (the reference's layout).
+__global__ void __launch_bounds__(kBlock) k_wang_landau(int8_t* lattice, const int8_t* J, uint64_t* H, double* logG,
+                                                        const double* factor, int* Earr, int* newE, uint64_t* off_iter,
+                                                        int* found, const int* start, const int* end,
+                                                        const int* off_hist, const int* off_lattice, const int8_t* cond,
+                                                        const int8_t* spectrum, const int* off_spec,
+                                                        const int* cond_sample, int X, int Y, int num_iterations,
+                                                        int num_lattices, int I, int per_sample, int seed, int boundary)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= num_lattices) {
+    return;
+  }
+  const int s = t / per_sample;
+  if (cond_sample[s] == -1) {
+    return;
+  }
+  const int W = per_sample / I;
+  const int k = t / W;
+  const int N = X * Y;
+  const int8_t* Js = J + static_cast<int64_t>(s) * 2 * N;
+  int8_t* lat = lattice + off_lattice[t];
+  const int sk = start[k];
+  const int ek = end[k];
+  const int emin = start[s * I];
+  const int emax = end[s * I + I - 1];
+  const int8_t* spec = spectrum + off_spec[s] - emin;
+  uint64_t* h = H + off_hist[t] - sk;
+  double* g = logG + off_hist[t] - sk;
+  const double lnf = wl_log(factor[t]);
+  const bool converged = cond[k] != 0;
+  stream_reader rng;
+  rng.init(seed + s, kTagWalk, static_cast<uint32_t>(t % per_sample), off_iter[t]);
+  int e_cur = Earr[t];
+  int flag = 0;
+  for (int it = 0; it < num_iterations; ++it) {
+    const int site = pick_index(rng.next(), N);
+    const int i = site / Y;
+    const int j = site - i * Y;
+    const int en = flip_energy_i8(lat, Js, X, Y, boundary, e_cur, i, j);
+    if (en < emin || en > emax || spec[en] != 1) {
+      flag = t + 1;
+      newE[t] = en;
+      break;
+    }
+    const bool in_range = en >= sk && en <= ek;
+    if (!converged) {
+      int upd = e_cur;
+      if (in_range) {
+        const double p = accept_prob(g[e_cur] - g[en]);
+        if (rng.next() < p) {
+          lat[site] = static_cast<int8_t>(-lat[site]);
+          e_cur = en;
+          upd = en;
+        }
+      }
+      h[upd] += 1;
+      g[upd] += lnf;
+    } else if (in_range) {
+      const double p = accept_prob(g[e_cur] - g[en]);
+      if (rng.next() < p) {
+        lat[site] = static_cast<int8_t>(-lat[site]);
+        e_cur = en;
+      }
+    }
+  }
+  Earr[t] = e_cur;
+  off_iter[t] = rng.draw;
+  if (num_iterations > 0) {
+    found[t] = flag;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------- mainrun loop -------
+
+namespace {
+
+// Per-round status reduced on the device: [0] = any new energy, [1..2] =
+// max factor bits.  One 16-byte D2H per round.
+__global__ void k_round_status(const int* found, const double* factor, int total, unsigned long long* status)
+{
+  __shared__ unsigned long long mx;
+  __shared__ int any;
+  if (threadIdx.x == 0) {
+    mx = 0;
+    any = 0;
+  }
+  __syncthreads();
+  unsigned long long local = 0;
+  int la = 0;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    la |= (found[t] != 0);
+    const unsigned long long b = dbl_bits(factor[t]);  // factors are positive: bit order = value order
+    local = (b > local) ? b : local;
+  }
+  atomicMax(&mx, local);
+  if (la) {
+    atomicOr(&any, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    status[0] = static_cast<unsigned long long>(any);
+    status[1] = mx;
+  }
+}
+
+__global__ void k_fill_zero(double* p, int64_t n)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < n) {
+    p[t] = 0.0;
+  }
+}
+
+}  // namespace
+
+}  // namespace wlamd
+
+using namespace wlamd;
+
+extern "C" {
+
+const char* wl_last_error(void)
+{
+  return g_last_error.c_str();
+}
+
+const char* wl_version(void)
+{
+  return "qec_pf_wang_landau_amd 0.1 (gfx950)";
+}
+
+int wl_calc_energy(int boundary, int* d_energy, const int8_t* d_lattice, const int8_t* d_interactions,
+                   const int* d_offset_lattice, int dim_X, int dim_Y, size_t num_lattices, int walker_per_interactions,
+                   void* stream)
+{
+  return guarded([&] {
+    require(dim_X > 1 && dim_Y > 1 && walker_per_interactions > 0, "calc_energy: bad dimensions");
+    if (num_lattices == 0) {
+      return;
+    }
+    k_calc_energy<<<grid_for(num_lattices, 128), 128, 0, as_stream(stream)>>>(
+      boundary, d_energy, d_lattice, d_interactions, d_offset_lattice, dim_X, dim_Y,
+      static_cast<int64_t>(num_lattices), walker_per_interactions);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_init_interactions(int8_t* d_interactions, size_t lattice_size, int num_interactions, int seed, float prob,
+                         void* stream)
+{
+  return guarded([&] {
+    const int64_t n = static_cast<int64_t>(lattice_size) * 2 * num_interactions;
+    if (n == 0) {
+      return;
+    }
+    k_init_interactions<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(
+      d_interactions, static_cast<int64_t>(lattice_size), num_interactions, seed, prob);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_apply_x_horizontal_error(int8_t* d_interactions, int X, int Y, int S, void* stream)
+{
+  return guarded([&] {
+    if (X * Y * S == 0) {
+      return;
+    }
+    k_apply_horizontal<<<grid_for(static_cast<int64_t>(Y) * S, 128), 128, 0, as_stream(stream)>>>(d_interactions, X,
+                                                                                                  Y, S);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_apply_x_vertical_error(int8_t* d_interactions, int X, int Y, int S, void* stream)
+{
+  return guarded([&] {
+    if (X * Y * S == 0) {
+      return;
+    }
+    k_apply_vertical<<<grid_for(static_cast<int64_t>(X) * S, 128), 128, 0, as_stream(stream)>>>(d_interactions, X,
+                                                                                                Y, S);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_init_lattice(int8_t* d_lattice, float* d_probs, int X, int Y, size_t num_lattices, int seed,
+                    int walker_per_interactions, void* stream)
+{
+  return guarded([&] {
+    require(walker_per_interactions > 0, "init_lattice: walker_per_interactions must be positive");
+    const int64_t n = static_cast<int64_t>(num_lattices) * X * Y;
+    if (n == 0) {
+      return;
+    }
+    k_init_lattice<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(d_lattice, d_probs, X, Y,
+                                                                      static_cast<int64_t>(num_lattices), seed,
+                                                                      walker_per_interactions);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_init_offsets_lattice(int* d_off, int X, int Y, size_t num_lattices, void* stream)
+{
+  return guarded([&] {
+    if (num_lattices == 0) {
+      return;
+    }
+    k_init_offsets_lattice<<<grid_for(num_lattices, 256), 256, 0, as_stream(stream)>>>(
+      d_off, X * Y, static_cast<int64_t>(num_lattices));
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_wang_landau_pre_run(int* d_energy, int* d_flag_found_interval, int8_t* d_lattice,
+                           int8_t* d_found_interval_lattice, uint64_t* d_histogram, uint64_t* d_seed_offset,
+                           const int8_t* d_interactions, const int* d_offset_lattice_per_walker, int E_min, int E_max,
+                           int num_wl_iterations, int dim_X, int dim_Y, int seed, unsigned int num_intervals,
+                           int len_interval, size_t num_total_walker, int walker_per_interactions, int boundary,
+                           void* stream)
+{
+  return guarded([&] {
+    require(walker_per_interactions > 0 && walker_per_interactions <= 1024,
+            "pre_run: walker_per_interactions must be in [1, 1024]");
+    require(num_total_walker % walker_per_interactions == 0, "pre_run: walkers not a multiple of walkers per sample");
+    require(len_interval > 0 && num_intervals > 0, "pre_run: bad prerun intervals");
+    require(E_min <= -2 * dim_X * dim_Y && E_max >= 2 * dim_X * dim_Y, "pre_run: energy range must cover [-2N, 2N]");
+    const int S = static_cast<int>(num_total_walker / walker_per_interactions);
+    if (S == 0) {
+      return;
+    }
+    const int len_hist = E_max - E_min + 1;
+    const size_t lds = sizeof(uint32_t) * len_hist + 2 * sizeof(int) * num_intervals;
+    require(lds <= 64 * 1024, "pre_run: histogram does not fit the workgroup's LDS");
+    k_prerun<<<S, round_up64(walker_per_interactions), lds, as_stream(stream)>>>(
+      d_energy, d_flag_found_interval, d_lattice, d_found_interval_lattice, d_histogram, d_seed_offset,
+      d_interactions, d_offset_lattice_per_walker, E_min, E_max, num_wl_iterations, dim_X, dim_Y, seed,
+      static_cast<int>(num_intervals), len_interval, walker_per_interactions, boundary);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_check_energy_ranges(int8_t* flags, const int* E, const int* start, const int* end, size_t total, int W,
+                           void* stream)
+{
+  return guarded([&] {
+    require(W > 0, "check_energy_ranges: walker_per_interval must be positive");
+    if (total == 0) {
+      return;
+    }
+    k_check_energy_ranges<<<grid_for(total, 256), 256, 0, as_stream(stream)>>>(flags, E, start, end,
+                                                                               static_cast<int64_t>(total), W);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_init_indices(int* idx, size_t total, int W, void* stream)
+{
+  return guarded([&] {
+    require(W > 0, "init_indices: walker_per_interval must be positive");
+    if (total == 0) {
+      return;
+    }
+    k_init_indices<<<grid_for(total, 256), 256, 0, as_stream(stream)>>>(idx, static_cast<int64_t>(total), W);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_reset_d_cond(int8_t* cond, const double* factor, int total_intervals, double beta, int W, void* stream)
+{
+  return guarded([&] {
+    if (total_intervals <= 0) {
+      return;
+    }
+    k_reset_d_cond<<<grid_for(total_intervals, 256), 256, 0, as_stream(stream)>>>(cond, factor, total_intervals, beta,
+                                                                                  W);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_init_offsets_histogram(int* off, const int* start, const int* end, const int* len_hist, int I, int total_walker,
+                              int W, void* stream)
+{
+  return guarded([&] {
+    require(I > 0 && W > 0, "init_offsets_histogram: bad shape");
+    if (total_walker <= 0) {
+      return;
+    }
+    k_init_offsets_histogram<<<grid_for(total_walker, 256), 256, 0, as_stream(stream)>>>(off, start, end, len_hist, I,
+                                                                                         total_walker, W);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_calc_average_log_g(double* d_shared_logG, const int* d_len_histograms, const double* d_log_G,
+                          const int* d_start, const int* d_end, const int8_t* d_spectrum, const int8_t* d_cond,
+                          const int* d_off_hist, const int* d_off_spec, const long long* d_off_shared,
+                          const int* d_cond_sample, int S, int W, int I, int total_len_histogram, void* stream)
+{
+  (void)d_len_histograms;
+  (void)total_len_histogram;
+  return guarded([&] {
+    require(I > 0 && W > 0, "calc_average_log_g: bad shape");
+    if (S <= 0) {
+      return;
+    }
+    k_calc_average<<<S * I, 64, 0, as_stream(stream)>>>(d_shared_logG, d_log_G, d_start, d_end, d_spectrum, d_cond,
+                                                        d_off_hist, d_off_spec, d_off_shared, d_cond_sample, W, I);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_check_interactions_finished(int* d_cond_sample, const int8_t* d_cond, const int* d_off_intervals, int I, int S,
+                                   void* stream)
+{
+  return guarded([&] {
+    if (S <= 0) {
+      return;
+    }
+    k_finished<<<grid_for(S, 256), 256, 0, as_stream(stream)>>>(d_cond_sample, d_cond, d_off_intervals, I, S);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_replica_exchange(int* d_offset_lattice, int* d_energy, int* d_idx, uint64_t* d_seed_offset, const int* d_start,
+                        const int* d_end, const int* d_off_hist, const int* d_cond_sample, const double* d_logG,
+                        int even, int seed, int I, int per_sample, int S, int flags, void* stream)
+{
+  return guarded([&] {
+    require(I > 0 && per_sample % I == 0, "replica_exchange: walker_per_interactions must be a multiple of I");
+    const int W = per_sample / I;
+    require(W <= 1024, "replica_exchange: at most 1024 walkers per interval");
+    if (S <= 0 || I < 2) {
+      return;
+    }
+    k_replica_exchange<<<S * I, round_up64(W), 0, as_stream(stream)>>>(d_offset_lattice, d_energy, d_idx,
+                                                                       d_seed_offset, d_start, d_end, d_off_hist,
+                                                                       d_cond_sample, d_logG, even, seed, I,
+                                                                       per_sample, flags & 1);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_check_histogram(uint64_t* d_H, double* d_factor, int8_t* d_cond, const int* d_off_hist, const int* d_end,
+                       const int* d_start, const int8_t* d_spectrum, const int* d_off_spec, const int* d_cond_sample,
+                       double alpha, int total, int per_sample, int I, void* stream)
+{
+  return guarded([&] {
+    require(I > 0 && per_sample % I == 0, "check_histogram: walker_per_interactions must be a multiple of I");
+    const int W = per_sample / I;
+    require(W <= 1024, "check_histogram: at most 1024 walkers per interval");
+    if (total <= 0) {
+      return;
+    }
+    k_check_histogram<<<total / W, round_up64(W), 0, as_stream(stream)>>>(d_H, d_factor, d_cond, d_off_hist, d_end,
+                                                                         d_start, d_spectrum, d_off_spec,
+                                                                         d_cond_sample, alpha, W, I);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_redistribute_g_values(double* d_log_G, const double* d_shared, const int* d_len_histograms, const int* d_start,
+                             const int* d_end, const int* d_cond_sample, const int* d_off_hist, const int8_t* d_cond,
+                             const long long* d_off_shared, int I, int W, int S, int total_len_histogram, void* stream)
+{
+  (void)d_len_histograms;
+  (void)total_len_histogram;
+  return guarded([&] {
+    require(I > 0 && W > 0, "redistribute_g_values: bad shape");
+    if (S <= 0) {
+      return;
+    }
+    k_redistribute<<<S * I, 256, 0, as_stream(stream)>>>(d_log_G, d_shared, d_start, d_end, d_cond_sample, d_off_hist,
+                                                         d_cond, d_off_shared, I, W);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_wang_landau(int8_t* d_lattice, const int8_t* d_J, uint64_t* d_H, double* d_logG, const double* factor,
+                   int* d_energy, int* d_newE, uint64_t* d_off_iter, int* foundFlag, const int* d_start,
+                   const int* d_end, const int* d_off_hist, const int* d_off_lattice, const int8_t* d_cond,
+                   const int8_t* d_spectrum, const int* d_off_spec, const int* d_cond_sample, int X, int Y,
+                   int num_iterations, int num_lattices, int I, int per_sample, int seed, int boundary, void* stream)
+{
+  return guarded([&] {
+    require(X > 1 && Y > 1, "wang_landau: lattice dimensions must be > 1");
+    require(I > 0 && per_sample % I == 0, "wang_landau: walker_per_interactions must be a multiple of I");
+    if (num_lattices <= 0) {
+      return;
+    }
+    k_wang_landau<<<grid_for(num_lattices, kBlock), kBlock, 0, as_stream(stream)>>>(
+      d_lattice, d_J, d_H, d_logG, factor, d_energy, d_newE, d_off_iter, foundFlag, d_start, d_end, d_off_hist,
+      d_off_lattice, d_cond, d_spectrum, d_off_spec, d_cond_sample, X, Y, num_iterations, num_lattices, I, per_sample,
+      seed, boundary);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_fisher_yates(int* d_shuffle, uint64_t* d_seed_offset, int seed, int total_intervals, int W, int per_sample,
+                    void* stream)
+{
+  return guarded([&] {
+    require(W > 0 && per_sample % W == 0, "fisher_yates: bad shape");
+    if (total_intervals <= 0) {
+      return;
+    }
+    k_fisher_yates<<<grid_for(total_intervals, 64), 64, 0, as_stream(stream)>>>(d_shuffle, d_seed_offset, seed,
+                                                                                total_intervals, W, per_sample);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_mainrun_loop(int8_t* d_lattice, const int8_t* d_J, uint64_t* d_H, double* d_logG, double* d_factor,
+                    int* d_energy, uint64_t* d_off_iter, int* d_off_lattice, int* d_idx, int8_t* d_cond,
+                    int* d_cond_sample, const int* d_start, const int* d_end, const int* d_off_hist,
+                    const int8_t* d_spectrum, const int* d_off_spec, const int* d_len_histograms, int S, int I, int W,
+                    int X, int Y, int boundary, int num_iterations, double alpha, double beta, int exchange_every,
+                    int seed, int flags, int64_t max_rounds, int64_t* counter_io, int64_t* rounds_out, void* stream)
+{
+  int status = 0;
+  const int rc = guarded([&] {
+    require(S > 0 && I > 0 && W > 0 && exchange_every > 0, "mainrun_loop: bad shape");
+    hipStream_t st = as_stream(stream);
+    const int total_walker = S * I * W;
+    const int per_sample = I * W;
+    const bool compat = (flags & 1) != 0;
+    // Host copies of the interval table (small) for the shared-ln g offsets.
+    std::vector<int> h_start(static_cast<size_t>(S) * I), h_end(static_cast<size_t>(S) * I);
+    WL_HIP(hipMemcpyAsync(h_start.data(), d_start, sizeof(int) * h_start.size(), hipMemcpyDeviceToHost, st));
+    WL_HIP(hipMemcpyAsync(h_end.data(), d_end, sizeof(int) * h_end.size(), hipMemcpyDeviceToHost, st));
+    WL_HIP(hipStreamSynchronize(st));
+    std::vector<long long> h_off_shared(static_cast<size_t>(S) * I);
+    std::vector<int> h_off_intervals(S + 1);
+    long long size_shared = 0;
+    for (int k = 0; k < S * I; ++k) {
+      h_off_shared[k] = size_shared;
+      size_shared += h_end[k] - h_start[k] + 1;
+    }
+    for (int s = 0; s <= S; ++s) {
+      h_off_intervals[s] = s * I;
+    }
+    double* d_shared = nullptr;
+    long long* d_off_shared = nullptr;
+    int* d_off_intervals = nullptr;
+    int* d_newE = nullptr;
+    int* d_found = nullptr;
+    unsigned long long* d_status = nullptr;
+    unsigned long long* h_status = nullptr;
+    auto cleanup = [&] {
+      (void)hipFree(d_shared);
+      (void)hipFree(d_off_shared);
+      (void)hipFree(d_off_intervals);
+      (void)hipFree(d_newE);
+      (void)hipFree(d_found);
+      (void)hipFree(d_status);
+      (void)hipHostFree(h_status);
+    };
+    try {
+      WL_HIP(hipMalloc(&d_shared, sizeof(double) * size_shared));
+      WL_HIP(hipMalloc(&d_off_shared, sizeof(long long) * h_off_shared.size()));
+      WL_HIP(hipMalloc(&d_off_intervals, sizeof(int) * h_off_intervals.size()));
+      WL_HIP(hipMalloc(&d_newE, sizeof(int) * total_walker));
+      WL_HIP(hipMalloc(&d_found, sizeof(int) * total_walker));
+      WL_HIP(hipMalloc(&d_status, 2 * sizeof(unsigned long long)));
+      WL_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_status), 2 * sizeof(unsigned long long), hipHostMallocDefault));
+      WL_HIP(hipMemcpyAsync(d_off_shared, h_off_shared.data(), sizeof(long long) * h_off_shared.size(),
+                            hipMemcpyHostToDevice, st));
+      WL_HIP(hipMemcpyAsync(d_off_intervals, h_off_intervals.data(), sizeof(int) * h_off_intervals.size(),
+                            hipMemcpyHostToDevice, st));
+      k_fill_zero<<<grid_for(size_shared, 256), 256, 0, st>>>(d_shared, size_shared);
+      WL_HIP(hipMemsetAsync(d_found, 0, sizeof(int) * total_walker, st));
+      k_round_status<<<1, 1024, 0, st>>>(d_found, d_factor, total_walker, d_status);
+      WL_HIP(hipMemcpyAsync(h_status, d_status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+      WL_HIP(hipStreamSynchronize(st));
+      double max_f = bits_dbl(h_status[1]);
+      const double ebeta = wl_exp(beta);
+      int64_t counter = (counter_io != nullptr) ? *counter_io : 1;
+      int64_t rounds = 0;
+      while (max_f - ebeta > 1e-10 && rounds < max_rounds) {
+        k_wang_landau<<<grid_for(total_walker, kBlock), kBlock, 0, st>>>(
+          d_lattice, d_J, d_H, d_logG, d_factor, d_energy, d_newE, d_off_iter, d_found, d_start, d_end, d_off_hist,
+          d_off_lattice, d_cond, d_spectrum, d_off_spec, d_cond_sample, X, Y, num_iterations, total_walker, I,
+          per_sample, seed, boundary);
+        k_check_histogram<<<S * I, round_up64(W), 0, st>>>(d_H, d_factor, d_cond, d_off_hist, d_end, d_start,
+                                                          d_spectrum, d_off_spec, d_cond_sample, alpha, W, I);
+        k_calc_average<<<S * I, 64, 0, st>>>(d_shared, d_logG, d_start, d_end, d_spectrum, d_cond, d_off_hist,
+                                             d_off_spec, d_off_shared, d_cond_sample, W, I);
+        k_redistribute<<<S * I, 256, 0, st>>>(d_logG, d_shared, d_start, d_end, d_cond_sample, d_off_hist, d_cond,
+                                              d_off_shared, I, W);
+        k_fill_zero<<<grid_for(size_shared, 256), 256, 0, st>>>(d_shared, size_shared);
+        k_reset_d_cond<<<grid_for(S * I, 256), 256, 0, st>>>(d_cond, d_factor, S * I, beta, W);
+        k_finished<<<grid_for(S, 256), 256, 0, st>>>(d_cond_sample, d_cond, d_off_intervals, I, S);
+        const bool exch = (counter % exchange_every == 0) && I > 1;
+        if (exch) {
+          k_replica_exchange<<<S * I, round_up64(W), 0, st>>>(d_off_lattice, d_energy, d_idx, d_off_iter, d_start,
+                                                             d_end, d_off_hist, d_cond_sample, d_logG, 1, seed, I,
+                                                             per_sample, compat ? 1 : 0);
+          k_replica_exchange<<<S * I, round_up64(W), 0, st>>>(d_off_lattice, d_energy, d_idx, d_off_iter, d_start,
+                                                             d_end, d_off_hist, d_cond_sample, d_logG, 0, seed, I,
+                                                             per_sample, compat ? 1 : 0);
+        }
+        // The status (new-energy flag from this round's sweep, max factor
+        // after the flatness update) is read once per round.  The sweep's
+        // flag must be taken before the exchange, but the exchange does not
+        // touch found/factor, so one read after it gives the same values.
+        k_round_status<<<1, 1024, 0, st>>>(d_found, d_factor, total_walker, d_status);
+        WL_LAUNCH_CHECK();
+        WL_HIP(hipMemcpyAsync(h_status, d_status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        WL_HIP(hipStreamSynchronize(st));
+        rounds += 1;
+        if (h_status[0] != 0) {
+          status = 3;
+          break;
+        }
+        max_f = bits_dbl(h_status[1]);
+        if (!compat) {
+          counter += 1;
+        }
+      }
+      if (rounds_out != nullptr) {
+        *rounds_out = rounds;
+      }
+      if (counter_io != nullptr) {
+        *counter_io = counter;
+      }
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+  if (rc != 0) {
+    return rc;
+  }
+  if (status == 3) {
+    set_last_error("mainrun: a walker found an energy outside the prerun spectrum");
+  }
+  return status;
+}
+
+}  // extern "C"

@@ -1,0 +1,447 @@
+"""Host orchestration of the replica-exchange Wang-Landau engine.
+
+Mirrors the reference's host drivers (src/prerun.cu, src/mainrun.cu) over the
+C ABI of libwl_amd: disorder and logical-error chains, prerun on the GPU,
+spectrum extraction, interval partition and seed lattices, the mainrun state
+in the reference's flat layout, the plain loop or the batched persistent
+kernel, then stitching and log Z per error class.
+
+Host arrays are numpy; device arrays are torch tensors (plumbing only: every
+computation on them is a libwl_amd kernel).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, lib
+
+ERROR_CLASSES = ("i", "v", "h", "c")
+
+TASK_DTYPE = np.dtype([
+    ("X", np.int32), ("Y", np.int32), ("boundary", np.int32), ("I", np.int32), ("W", np.int32),
+    ("rng_key", np.int32), ("num_iterations", np.int32), ("exchange_every", np.int32),
+    ("energy_stride", np.int32), ("parity_base", np.int32), ("flags", np.int32),
+    ("interval_off", np.int32), ("spectrum_off", np.int32), ("pad_", np.int32),
+    ("walker_off", np.int64), ("hist_off", np.int64), ("lattice_off", np.int64), ("couplings_off", np.int64),
+    ("alpha", np.float64), ("beta", np.float64),
+], align=True)
+assert TASK_DTYPE.itemsize == 104
+
+
+def _p(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+# --------------------------------------------------------------- host logic --
+
+@dataclass
+class Intervals:
+    start: np.ndarray
+    end: np.ndarray
+    len_histogram_over_all_walkers: int
+    len_interval: int
+
+
+def interval_results(e_min: int, e_max: int, num_intervals: int, num_walkers: int, overlap: float) -> Intervals:
+    """src/interval.cpp:5-41 (raises WLError/status 1 where the reference throws)."""
+    start = np.zeros(num_intervals, np.int32)
+    end = np.zeros(num_intervals, np.int32)
+    tot = C.c_int64()
+    ln = C.c_int()
+    call("wl_interval_results", e_min, e_max, num_intervals, num_walkers, C.c_float(overlap), _p(start), _p(end),
+         C.byref(tot), C.byref(ln))
+    return Intervals(start, end, tot.value, ln.value)
+
+
+def energy_spectrum(hist_binary: np.ndarray, X: int, Y: int) -> tuple[int, int, np.ndarray]:
+    """src/file_io.cpp:558-616 for one sample: (E_min, E_max, 0/1 mask)."""
+    h = np.ascontiguousarray(hist_binary, dtype=np.uint8)
+    mask = np.zeros(h.size, np.int8)
+    emin, emax, ln = C.c_int(), C.c_int(), C.c_int()
+    call("wl_energy_spectrum", _p(h), h.size, X, Y, C.byref(emin), C.byref(emax), _p(mask), C.byref(ln))
+    return emin.value, emax.value, mask[: ln.value].copy()
+
+
+def energy_stride(mask: np.ndarray) -> int:
+    m = np.ascontiguousarray(mask, dtype=np.int8)
+    return int(lib().wl_energy_stride(_p(m), m.size))
+
+
+def select_seed_lattices(found: np.ndarray, found_energy: np.ndarray, start: np.ndarray, end: np.ndarray,
+                         W: int, N: int) -> np.ndarray:
+    """src/file_io.cpp:618-659 for one sample: I*W seed lattices."""
+    f = np.ascontiguousarray(found, dtype=np.int8)
+    fe = np.ascontiguousarray(found_energy, dtype=np.int32)
+    st = np.ascontiguousarray(start, dtype=np.int32)
+    en = np.ascontiguousarray(end, dtype=np.int32)
+    out = np.zeros(st.size * W * N, np.int8)
+    call("wl_select_seed_lattices", _p(f), _p(fe), fe.size, N, _p(st), _p(en), st.size, W, _p(out))
+    return out
+
+
+def stitch(logG_sample: np.ndarray, start: np.ndarray, end: np.ndarray, W: int, X: int, Y: int
+           ) -> tuple[np.ndarray, np.ndarray]:
+    """src/result_handling.cpp:368-431 without the file write: (E, ln g)."""
+    g = np.ascontiguousarray(logG_sample, dtype=np.float64)
+    st = np.ascontiguousarray(start, dtype=np.int32)
+    en = np.ascontiguousarray(end, dtype=np.int32)
+    cap = int(en[-1] - st[0] + 1)
+    oe = np.zeros(cap, np.int32)
+    og = np.zeros(cap, np.float64)
+    n = C.c_int()
+    call("wl_stitch_log_g", _p(g), _p(st), _p(en), st.size, W, X, Y, _p(oe), _p(og), cap, C.byref(n))
+    return oe[: n.value].copy(), og[: n.value].copy()
+
+
+def log_partition(E: np.ndarray, lng: np.ndarray, beta: float) -> float:
+    e = np.ascontiguousarray(E, dtype=np.int32)
+    g = np.ascontiguousarray(lng, dtype=np.float64)
+    return float(lib().wl_log_partition(_p(e), _p(g), e.size, beta))
+
+
+def nishimori_beta(p: float) -> float:
+    return float(lib().wl_nishimori_beta(p))
+
+
+def lattice_energies(lattices: np.ndarray, J: np.ndarray, X: int, Y: int, boundary: int) -> np.ndarray:
+    """Energies of lattices (n, X*Y) under couplings J (2*X*Y), E = -sum J s s'."""
+    N = X * Y
+    s = lattices.reshape(-1, X, Y).astype(np.int32)
+    Jh = J[:N].reshape(X, Y).astype(np.int32)
+    Jv = J[N:].reshape(X, Y).astype(np.int32)
+    right = s * np.roll(s, -1, axis=2) * Jh
+    down = s * np.roll(s, -1, axis=1) * Jv
+    if boundary == 1:
+        right[:, :, Y - 1] = 0
+        down[:, X - 1, :] = 0
+    return -(right.sum(axis=(1, 2)) + down.sum(axis=(1, 2))).astype(np.int32)
+
+
+# ------------------------------------------------------------ state layout --
+
+@dataclass
+class RunConfig:
+    X: int
+    Y: int
+    boundary: int = 0
+    num_intervals: int = 4
+    walker_per_interval: int = 4
+    overlap: float = 0.25
+    alpha: float = 0.8
+    beta: float = 1e-6
+    num_iterations: int = 1000
+    exchange_every: int = 20
+    seed: int = 12345
+    compat: bool = False
+
+
+@dataclass
+class SampleInput:
+    """One (sample, class) task's inputs after the prerun."""
+    J: np.ndarray              # int8 [2N]
+    e_min: int
+    e_max: int
+    mask: np.ndarray           # int8 spectrum flags over [e_min, e_max]
+    found: np.ndarray          # int8 [n_found, N] prerun lattices
+    found_energy: np.ndarray   # int32 [n_found]
+    rng_key: int               # seed + sample
+
+
+@dataclass
+class MainrunState:
+    """Reference-layout state of S independent samples (src/mainrun.cu:42-165)."""
+    cfg: RunConfig
+    S: int
+    J: np.ndarray
+    lattice: np.ndarray
+    off_lattice: np.ndarray
+    E: np.ndarray
+    off_iter: np.ndarray
+    logG: np.ndarray
+    H: np.ndarray
+    factor: np.ndarray
+    idx: np.ndarray
+    cond: np.ndarray
+    cond_sample: np.ndarray
+    start: np.ndarray
+    end: np.ndarray
+    off_hist: np.ndarray
+    spectrum: np.ndarray
+    off_spec: np.ndarray
+    len_hist: np.ndarray
+    keys: np.ndarray
+    strides: np.ndarray
+    counter: np.ndarray = field(default=None)
+    rounds: np.ndarray = field(default=None)
+    status: np.ndarray = field(default=None)
+
+    @property
+    def N(self) -> int:
+        return self.cfg.X * self.cfg.Y
+
+    def copy(self) -> "MainrunState":
+        out = MainrunState(**{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.__dict__.items()})
+        return out
+
+    def sample_logG(self, s: int) -> np.ndarray:
+        base = int(self.off_hist[s * self.cfg.num_intervals * self.cfg.walker_per_interval])
+        return self.logG[base: base + int(self.len_hist[s])]
+
+    def sample_intervals(self, s: int) -> tuple[np.ndarray, np.ndarray]:
+        I = self.cfg.num_intervals
+        return self.start[s * I:(s + 1) * I], self.end[s * I:(s + 1) * I]
+
+
+def build_state(samples: list[SampleInput], cfg: RunConfig) -> MainrunState:
+    """Set up S samples exactly as src/mainrun.cu:42-228 does: intervals per
+    sample, seed lattices, initial energies (checked against the intervals),
+    factor e, zero histograms, identity replica indices."""
+    X, Y, I, W = cfg.X, cfg.Y, cfg.num_intervals, cfg.walker_per_interval
+    N = X * Y
+    S = len(samples)
+    starts, ends, lens, lattices, masks, strides = [], [], [], [], [], []
+    for smp in samples:
+        iv = interval_results(smp.e_min, smp.e_max, I, W, cfg.overlap)
+        starts.append(iv.start)
+        ends.append(iv.end)
+        lens.append(iv.len_histogram_over_all_walkers)
+        lattices.append(select_seed_lattices(smp.found, smp.found_energy, iv.start, iv.end, W, N))
+        masks.append(np.asarray(smp.mask, np.int8))
+        strides.append(energy_stride(smp.mask))
+    start = np.concatenate(starts).astype(np.int32)
+    end = np.concatenate(ends).astype(np.int32)
+    len_hist = np.array(lens, np.int32)
+    total_walker = S * I * W
+    off_hist = np.zeros(total_walker, np.int32)
+    base = 0
+    for s in range(S):
+        len0 = end[s * I] - start[s * I] + 1
+        for k in range(I):
+            lenk = end[s * I + k] - start[s * I + k] + 1
+            for w in range(W):
+                off_hist[(s * I + k) * W + w] = base + k * W * len0 + w * lenk
+        base += int(len_hist[s])
+    spectrum = np.concatenate(masks).astype(np.int8)
+    off_spec = np.zeros(S, np.int32)
+    off_spec[1:] = np.cumsum([m.size for m in masks])[:-1]
+    J = np.concatenate([np.asarray(smp.J, np.int8) for smp in samples])
+    lattice = np.concatenate(lattices).astype(np.int8)
+    E = np.concatenate([
+        lattice_energies(lattices[s].reshape(I * W, N), samples[s].J, X, Y, cfg.boundary) for s in range(S)])
+    for s in range(S):
+        for k in range(I):
+            e = E[(s * I + k) * W:(s * I + k + 1) * W]
+            if np.any(e < start[s * I + k]) or np.any(e > end[s * I + k]):
+                raise RuntimeError("Calculated energies do not match intervals")
+    return MainrunState(
+        cfg=cfg, S=S, J=J, lattice=lattice,
+        off_lattice=(np.arange(total_walker, dtype=np.int64) * N).astype(np.int32),
+        E=E.astype(np.int32), off_iter=np.zeros(total_walker, np.uint64),
+        logG=np.zeros(int(len_hist.sum()), np.float64), H=np.zeros(int(len_hist.sum()), np.uint64),
+        factor=np.full(total_walker, np.e, np.float64),
+        idx=np.tile(np.arange(W, dtype=np.int32), S * I), cond=np.zeros(S * I, np.int8),
+        cond_sample=np.zeros(S, np.int32), start=start, end=end, off_hist=off_hist, spectrum=spectrum,
+        off_spec=off_spec, len_hist=len_hist, keys=np.array([smp.rng_key for smp in samples], np.int32),
+        strides=np.array(strides, np.int32), counter=np.ones(S, np.int64), rounds=np.zeros(S, np.int64),
+        status=np.zeros(S, np.int32))
+
+
+def make_tasks(state: MainrunState) -> np.ndarray:
+    """Task table of the batched form: task s = sample s of the state."""
+    cfg = state.cfg
+    I, W, N = cfg.num_intervals, cfg.walker_per_interval, state.N
+    t = np.zeros(state.S, TASK_DTYPE)
+    for s in range(state.S):
+        t[s]["X"], t[s]["Y"], t[s]["boundary"] = cfg.X, cfg.Y, cfg.boundary
+        t[s]["I"], t[s]["W"] = I, W
+        t[s]["rng_key"] = state.keys[s]
+        t[s]["num_iterations"] = cfg.num_iterations
+        t[s]["exchange_every"] = cfg.exchange_every
+        stride = int(state.strides[s])
+        t[s]["energy_stride"] = stride if stride in (1, 2, 4) else 1
+        t[s]["parity_base"] = s * I
+        t[s]["flags"] = 1 if cfg.compat else 0
+        t[s]["interval_off"] = s * I
+        t[s]["spectrum_off"] = state.off_spec[s]
+        t[s]["walker_off"] = s * I * W
+        t[s]["hist_off"] = state.off_hist[s * I * W]
+        t[s]["lattice_off"] = s * I * W * N
+        t[s]["couplings_off"] = s * 2 * N
+        t[s]["alpha"], t[s]["beta"] = cfg.alpha, cfg.beta
+    return t
+
+
+@dataclass
+class TaskBatch:
+    """Tasks of possibly different shapes in one set of flat arrays."""
+    tasks: np.ndarray
+    arrays: dict
+    parts: list  # (state, first task, first walker, first hist, first lattice byte, first interval, first spec byte)
+
+    def split(self) -> list[MainrunState]:
+        """Per-part states (views copied out) in the layout of their source state."""
+        out = []
+        for st, q0, w0, h0, l0, i0, s0 in self.parts:
+            I, W, N = st.cfg.num_intervals, st.cfg.walker_per_interval, st.N
+            nw, nh, ni = st.S * I * W, st.logG.size, st.S * I
+            part = st.copy()
+            a = self.arrays
+            part.lattice = a["lattice"][l0:l0 + nw * N].copy()
+            part.off_lattice = (a["off_lattice"][w0:w0 + nw] - l0).astype(np.int32)
+            for k in ("E", "off_iter", "factor", "idx"):
+                setattr(part, k, a[k][w0:w0 + nw].copy())
+            part.logG = a["logG"][h0:h0 + nh].copy()
+            part.H = a["H"][h0:h0 + nh].copy()
+            part.cond = a["cond"][i0:i0 + ni].copy()
+            for k in ("cond_sample", "counter", "rounds", "status"):
+                setattr(part, k, a[k][q0:q0 + st.S].copy())
+            out.append(part)
+        return out
+
+
+def merge_states(states: list[MainrunState]) -> TaskBatch:
+    """Concatenate states (each sample = one task) into one task batch."""
+    names = ("J", "lattice", "off_lattice", "E", "off_iter", "logG", "H", "factor", "idx", "cond", "cond_sample",
+             "start", "end", "spectrum", "counter", "rounds", "status")
+    acc = {k: [] for k in names}
+    tasks = []
+    parts = []
+    q0 = w0 = h0 = l0 = i0 = s0 = j0 = 0
+    for st in states:
+        I, W, N = st.cfg.num_intervals, st.cfg.walker_per_interval, st.N
+        t = make_tasks(st)
+        t["walker_off"] += w0
+        t["hist_off"] += h0
+        t["lattice_off"] += l0
+        t["interval_off"] += i0
+        t["spectrum_off"] += s0
+        t["couplings_off"] += j0
+        t["parity_base"] += i0
+        tasks.append(t)
+        parts.append((st, q0, w0, h0, l0, i0, s0))
+        for k in names:
+            v = getattr(st, k)
+            if k == "off_lattice":
+                v = (v.astype(np.int64) + l0).astype(np.int32)
+            acc[k].append(v)
+        q0 += st.S
+        w0 += st.S * I * W
+        h0 += st.logG.size
+        l0 += st.lattice.size
+        i0 += st.S * I
+        s0 += st.spectrum.size
+        j0 += st.J.size
+    arrays = {k: np.concatenate(v) for k, v in acc.items()}
+    return TaskBatch(np.concatenate(tasks), arrays, parts)
+
+
+class BatchDevice:
+    """Device copy of a TaskBatch."""
+
+    def __init__(self, batch: TaskBatch):
+        torch = _torch()
+        self.batch = batch
+        self.t = {k: to_device(v) for k, v in batch.arrays.items()}
+        self.d_tasks = torch.from_numpy(batch.tasks.view(np.uint8).copy()).cuda()
+
+    def __getitem__(self, name):
+        return dptr(self.t[name])
+
+    def run(self, max_rounds: int = 1 << 40) -> None:
+        b = self.batch
+        call("wl_fused_run", dptr(self.d_tasks), _p(b.tasks), b.tasks.size, self["lattice"], self["J"], self["H"],
+             self["logG"], self["factor"], self["E"], self["off_iter"], self["off_lattice"], self["idx"],
+             self["cond"], self["cond_sample"], self["start"], self["end"], self["spectrum"], self["counter"],
+             self["rounds"], self["status"], max_rounds, None)
+
+    def pull(self) -> TaskBatch:
+        arrays = {k: to_host(v, self.batch.arrays[k].dtype) for k, v in self.t.items()}
+        return TaskBatch(self.batch.tasks, arrays, self.batch.parts)
+
+
+def task_lds_bytes(task: np.ndarray, start: np.ndarray, end: np.ndarray) -> int:
+    tk = np.ascontiguousarray(task)
+    st = np.ascontiguousarray(start, np.int32)
+    en = np.ascontiguousarray(end, np.int32)
+    return int(lib().wl_fused_task_lds_bytes(_p(tk), _p(st), _p(en)))
+
+
+# ------------------------------------------------------------ device side --
+
+_STATE_ARRAYS = ("J", "lattice", "off_lattice", "E", "off_iter", "logG", "H", "factor", "idx", "cond",
+                 "cond_sample", "start", "end", "off_hist", "spectrum", "off_spec", "len_hist", "counter", "rounds",
+                 "status")
+
+
+def _torch():
+    import torch  # noqa: PLC0415  (plumbing for device memory only)
+    return torch
+
+
+def to_device(a: np.ndarray):
+    torch = _torch()
+    if a.dtype == np.uint64:
+        t = torch.from_numpy(a.view(np.int64)).cuda()
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return t
+
+
+def to_host(t, dtype) -> np.ndarray:
+    a = t.cpu().numpy()
+    if dtype == np.uint64:
+        return a.view(np.uint64).copy()
+    return a.astype(dtype, copy=False).copy()
+
+
+def dptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+class DeviceState:
+    """Device copy of a MainrunState (one torch tensor per array)."""
+
+    def __init__(self, state: MainrunState):
+        self.host = state
+        self.t = {name: to_device(getattr(state, name)) for name in _STATE_ARRAYS}
+
+    def __getitem__(self, name):
+        return dptr(self.t[name])
+
+    def pull(self) -> MainrunState:
+        out = self.host.copy()
+        for name in _STATE_ARRAYS:
+            setattr(out, name, to_host(self.t[name], getattr(self.host, name).dtype))
+        return out
+
+
+def run_plain(ds: DeviceState, max_rounds: int = 1 << 40) -> tuple[int, int]:
+    """wl_mainrun_loop over all samples of the state: (status, rounds)."""
+    st = ds.host
+    cfg = st.cfg
+    counter = C.c_int64(int(st.counter[0]))
+    rounds = C.c_int64(0)
+    rc = lib().wl_mainrun_loop(
+        ds["lattice"], ds["J"], ds["H"], ds["logG"], ds["factor"], ds["E"], ds["off_iter"], ds["off_lattice"],
+        ds["idx"], ds["cond"], ds["cond_sample"], ds["start"], ds["end"], ds["off_hist"], ds["spectrum"],
+        ds["off_spec"], ds["len_hist"], st.S, cfg.num_intervals, cfg.walker_per_interval, cfg.X, cfg.Y,
+        cfg.boundary, cfg.num_iterations, cfg.alpha, cfg.beta, cfg.exchange_every, int(st.keys[0]),
+        1 if cfg.compat else 0, max_rounds, C.byref(counter), C.byref(rounds), None)
+    if rc not in (0, 3):
+        raise _lib.WLError("wl_mainrun_loop", rc, _lib.last_error())
+    return rc, rounds.value
+
+
+def run_fused(ds: DeviceState, tasks: np.ndarray, max_rounds: int = 1 << 40, d_tasks=None) -> None:
+    """wl_fused_run over the task table."""
+    torch = _torch()
+    if d_tasks is None:
+        d_tasks = torch.from_numpy(tasks.view(np.uint8).copy()).cuda()
+    call("wl_fused_run", dptr(d_tasks), _p(tasks), tasks.size, ds["lattice"], ds["J"], ds["H"], ds["logG"],
+         ds["factor"], ds["E"], ds["off_iter"], ds["off_lattice"], ds["idx"], ds["cond"], ds["cond_sample"],
+         ds["start"], ds["end"], ds["spectrum"], ds["counter"], ds["rounds"], ds["status"], max_rounds, None)

@@ -1,0 +1,123 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libwl_oracle.so) and of the
+reference's own interval partition (oracle/_ref/libwl_ref.so).  Test
+infrastructure only."""
+
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "_build" / "libwl_oracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libwl_ref.so"
+
+P = C.c_void_p
+I32 = C.c_int
+U32 = C.c_uint32
+I64 = C.c_int64
+U64 = C.c_uint64
+F32 = C.c_float
+F64 = C.c_double
+
+SIG = {
+    "wlo_philox4x32_10": (None, [P, P, P]),
+    "wlo_uniform": (F64, [I32, U32, U32, U64]),
+    "wlo_pick": (I32, [F64, I32]),
+    "wlo_exp": (F64, [F64]),
+    "wlo_log": (F64, [F64]),
+    "wlo_sqrt": (F64, [F64]),
+    "wlo_calc_energy": (None, [I32, P, P, P, P, I32, I32, I64, I32]),
+    "wlo_init_interactions": (None, [P, I64, I32, I32, F32]),
+    "wlo_apply_x_horizontal_error": (None, [P, I32, I32, I32]),
+    "wlo_apply_x_vertical_error": (None, [P, I32, I32, I32]),
+    "wlo_init_lattice": (None, [P, P, I32, I32, I64, I32, I32]),
+    "wlo_init_offsets_lattice": (None, [P, I32, I32, I64]),
+    "wlo_wang_landau_pre_run": (None, [P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I64, I32, I32]),
+    "wlo_check_energy_ranges": (None, [P, P, P, P, I64, I32]),
+    "wlo_init_indices": (None, [P, I64, I32]),
+    "wlo_reset_d_cond": (None, [P, P, I32, F64, I32]),
+    "wlo_init_offsets_histogram": (None, [P, P, P, P, I32, I32, I32]),
+    "wlo_check_interactions_finished": (None, [P, P, I32, I32]),
+    "wlo_calc_average_log_g": (None, [P, P, P, P, P, P, P, P, P, P, P, I32, I32, I32, I32]),
+    "wlo_redistribute_g_values": (None, [P, P, P, P, P, P, P, P, P, I32, I32, I32, I32]),
+    "wlo_replica_exchange": (None, [P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32]),
+    "wlo_check_histogram": (None, [P, P, P, P, P, P, P, P, P, F64, I32, I32, I32]),
+    "wlo_wang_landau": (None, [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                               I32, I32, I32, I32, I32, I32, I32, I32]),
+    "wlo_interval_results": (I32, [I32, I32, U32, I32, F32, P, P, P, P]),
+    "wlo_mainrun_loop": (I32, [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                               I32, I32, I32, I32, I32, I32, I32, F64, F64, I32, I32, I32, I64, P]),
+    "wlo_stitch": (I32, [P, P, P, I32, I32, I32, I32, P, P, I32]),
+    "wlo_log_partition": (F64, [P, P, I32, F64]),
+    "wlo_nishimori_beta": (F64, [F64]),
+    "wlo_run_tasks": (I32, [I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                            I32, I32, I32, I32, I32, I32, F64, F64, I32, I64, P, I32]),
+    "wlo_max_threads": (I32, []),
+    "wlo_exact_dos_bruteforce": (I32, [P, I32, I32, I32, P]),
+    "wlo_exact_dos_transfer": (I32, [P, I32, I32, I32, P, P]),
+}
+
+_O = None
+_R = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+
+
+def oracle() -> C.CDLL:
+    global _O
+    if _O is None:
+        if not ORACLE_SO.exists():
+            build()
+        lib = C.CDLL(str(ORACLE_SO))
+        for name, (res, args) in SIG.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _O = lib
+    return _O
+
+
+def ref() -> C.CDLL | None:
+    """The reference's interval partition, or None where /root/reference was absent at build time."""
+    global _R
+    if _R is None and REF_SO.exists():
+        lib = C.CDLL(str(REF_SO))
+        lib.wlref_interval_results.restype = I32
+        lib.wlref_interval_results.argtypes = [I32, I32, U32, I32, F32, P, P, P, P]
+        _R = lib
+    return _R
+
+
+def p(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def exact_lng(J: np.ndarray, X: int, Y: int, boundary: int) -> np.ndarray:
+    """Exact ln g(E) over [-2N, 2N] (-inf where g = 0) from the transfer matrix."""
+    N = X * Y
+    lng = np.zeros(4 * N + 1)
+    cnt = np.zeros(4 * N + 1, np.uint64)
+    j = np.ascontiguousarray(J, np.int8)
+    rc = oracle().wlo_exact_dos_transfer(p(j), X, Y, boundary, p(lng), p(cnt))
+    assert rc == 0
+    return lng
+
+
+def run_loop(st, max_rounds: int = 1 << 40, counter: int | None = None) -> int:
+    """wlo_mainrun_loop on a MainrunState (in place); returns rounds or -1."""
+    cfg = st.cfg
+    c = np.array([st.counter[0] if counter is None else counter], np.int64)
+    r = oracle().wlo_mainrun_loop(
+        p(st.lattice), p(st.J), p(st.H), p(st.logG), p(st.factor), p(st.E), p(st.off_iter), p(st.off_lattice),
+        p(st.idx), p(st.cond), p(st.cond_sample), p(st.start), p(st.end), p(st.off_hist), p(st.spectrum),
+        p(st.off_spec), p(st.len_hist), st.S, cfg.num_intervals, cfg.walker_per_interval, cfg.X, cfg.Y,
+        cfg.boundary, cfg.num_iterations, cfg.alpha, cfg.beta, cfg.exchange_every, int(st.keys[0]),
+        1 if cfg.compat else 0, max_rounds, p(c))
+    st.counter[:] = c[0]
+    return r

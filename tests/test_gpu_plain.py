@@ -1,0 +1,292 @@
+"""GPU parity of the plain kernels (include/wl_amd.h part 1) and of the plain
+mainrun loop against the CPU oracle: bit-exact for every integer, byte and
+FP64 array (all FP64 updates run the same operation sequence on both sides)."""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from helpers import exact_samples, oracle_disorder, oracle_samples
+from oracle_lib import oracle, p, run_loop
+from qec_pf_wang_landau_amd import engine
+from qec_pf_wang_landau_amd._lib import call, lib
+from qec_pf_wang_landau_amd.engine import dptr, to_device, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def d(a):
+    return to_device(np.ascontiguousarray(a))
+
+
+def h(t, like):
+    return to_host(t, like.dtype)
+
+
+def test_library_version(cuda):
+    assert b"gfx950" in lib().wl_version()
+
+
+def test_math_host_device_bit_exact(cuda):
+    """exp/log/sqrt and the Philox stream on the device equal the oracle's bits:
+    exercised through the prerun acceptance and the flatness square root."""
+    f = np.full(64, math.e)
+    for _ in range(30):
+        tf = d(f)
+        cond = d(np.zeros(1, np.int8))
+        H = d(np.ones(64, np.uint64))
+        oh = d(np.zeros(64, np.int32))
+        st = d(np.array([0], np.int32))
+        en = d(np.array([0], np.int32))
+        spec = d(np.ones(1, np.int8))
+        ofs = d(np.zeros(1, np.int32))
+        cs = d(np.zeros(1, np.int32))
+        call("wl_check_histogram", dptr(H), dptr(tf), dptr(cond), dptr(oh), dptr(en), dptr(st), dptr(spec),
+             dptr(ofs), dptr(cs), C.c_double(0.8), 64, 64, 1, None)
+        got = h(tf, f)
+        expect = np.array([oracle().wlo_sqrt(float(x)) for x in f])
+        assert np.array_equal(got, expect)
+        f = got
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_calc_energy_known_answer_gpu(cuda, boundary):
+    lat = np.array([1, 1, -1, 1, -1, -1, 1, -1, 1, -1, 1, -1, -1, -1, -1, -1], np.int8)
+    J = np.array([1, -1, 1, 1, 1, 1, 1, 1, -1, 1, 1, -1, 1, 1, 1, 1,
+                  1, 1, -1, 1, 1, 1, 1, 1, -1, 1, 1, 1, 1, 1, 1, 1], np.int8)
+    E = d(np.zeros(1, np.int32))
+    call("wl_calc_energy", boundary, dptr(E), dptr(d(lat)), dptr(d(J)), dptr(d(np.zeros(1, np.int32))), 4, 4, 1, 1,
+         None)
+    assert h(E, np.zeros(1, np.int32))[0] == (-6 if boundary == 0 else -4)
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_calc_energy_random_vs_oracle(cuda, boundary):
+    rng = np.random.default_rng(1)
+    X, Y, n, per = 7, 5, 300, 30
+    lat = rng.choice(np.array([-1, 1], np.int8), n * X * Y)
+    J = rng.choice(np.array([-1, 1], np.int8), (n // per) * 2 * X * Y)
+    off = (np.arange(n) * X * Y).astype(np.int32)
+    E = np.zeros(n, np.int32)
+    oracle().wlo_calc_energy(boundary, p(E), p(lat), p(J), p(off), X, Y, n, per)
+    tE = d(np.zeros(n, np.int32))
+    call("wl_calc_energy", boundary, dptr(tE), dptr(d(lat)), dptr(d(J)), dptr(d(off)), X, Y, n, per, None)
+    assert np.array_equal(h(tE, E), E)
+
+
+@pytest.mark.parametrize("cls", ["i", "h", "v", "c"])
+def test_disorder_and_chains_vs_oracle(cuda, cls):
+    X, Y, S = 6, 5, 7
+    J = oracle_disorder(77, S, X, Y, 0.13, cls)
+    tJ = d(np.zeros(S * 2 * X * Y, np.int8))
+    call("wl_init_interactions", dptr(tJ), X * Y, S, 77, C.c_float(0.13), None)
+    if cls in ("h", "c"):
+        call("wl_apply_x_horizontal_error", dptr(tJ), X, Y, S, None)
+    if cls in ("v", "c"):
+        call("wl_apply_x_vertical_error", dptr(tJ), X, Y, S, None)
+    assert np.array_equal(h(tJ, J), J)
+
+
+def test_init_lattice_vs_oracle(cuda):
+    X, Y, n, per = 6, 6, 64, 16
+    lat = np.zeros(n * X * Y, np.int8)
+    probs = np.zeros(n, np.float32)
+    oracle().wlo_init_lattice(p(lat), p(probs), X, Y, n, 9, per)
+    tl = d(np.zeros_like(lat))
+    tp = d(np.zeros_like(probs))
+    call("wl_init_lattice", dptr(tl), dptr(tp), X, Y, n, 9, per, None)
+    assert np.array_equal(h(tl, lat), lat)
+    assert np.array_equal(h(tp, probs), probs)
+
+
+@pytest.mark.parametrize("boundary,X,Y,prob", [(0, 4, 4, 0.1), (1, 5, 6, 0.2), (0, 6, 6, 0.04)])
+def test_prerun_vs_oracle(cuda, boundary, X, Y, prob):
+    S, Wp, iters, Ipre = 3, 64, 500, 8
+    N = X * Y
+    J = oracle_disorder(5, S, X, Y, prob, "v")
+    iv = engine.interval_results(-2 * N, 2 * N, Ipre, 1, 1.0)
+    total = S * Wp
+    lat = np.zeros(total * N, np.int8)
+    oracle().wlo_init_lattice(p(lat), p(np.zeros(total, np.float32)), X, Y, total, 5, Wp)
+    off = (np.arange(total) * N).astype(np.int32)
+    E = np.zeros(total, np.int32)
+    oracle().wlo_calc_energy(boundary, p(E), p(lat), p(J), p(off), X, Y, total, Wp)
+    dev = {k: d(v) for k, v in dict(E=E, lat=lat, J=J, off=off).items()}
+    hist = np.zeros(S * (4 * N + 1), np.uint64)
+    so = np.zeros(total, np.uint64)
+    flag = np.zeros(S * Ipre, np.int32)
+    found = np.zeros(S * Ipre * N, np.int8)
+    tH, tso, tflag, tfound = d(hist), d(so), d(flag), d(found)
+    oracle().wlo_wang_landau_pre_run(p(E), p(flag), p(lat), p(found), p(hist), p(so), p(J), p(off), -2 * N, 2 * N,
+                                     iters, X, Y, 5, Ipre, iv.len_interval, total, Wp, boundary)
+    call("wl_wang_landau_pre_run", dptr(dev["E"]), dptr(tflag), dptr(dev["lat"]), dptr(tfound), dptr(tH), dptr(tso),
+         dptr(dev["J"]), dptr(dev["off"]), -2 * N, 2 * N, iters, X, Y, 5, Ipre, iv.len_interval, total, Wp, boundary,
+         None)
+    assert np.array_equal(h(tH, hist), hist)
+    assert np.array_equal(h(dev["E"], E), E)
+    assert np.array_equal(h(dev["lat"], lat), lat)
+    assert np.array_equal(h(tso, so), so)
+    assert np.array_equal(h(tflag, flag), flag)
+    assert np.array_equal(h(tfound, found), found)
+
+
+# --------------------------------------------------------- reference goldens
+
+from test_oracle import (G_END, G_LEN, G_OFF_HIST, G_OFF_SHARED, G_OFF_SPEC, G_SPEC, G_START,  # noqa: E402
+                         REDIST_EXPECT, REDIST_SHARED, calc_average_expect, check_histogram_golden_expect,
+                         check_histogram_golden_inputs)
+
+
+def test_check_histogram_golden_gpu(cuda):
+    H, f, cond, cs = check_histogram_golden_inputs()
+    tH, tf, tc = d(H), d(f), d(cond)
+    call("wl_check_histogram", dptr(tH), dptr(tf), dptr(tc), dptr(d(G_OFF_HIST)), dptr(d(G_END)), dptr(d(G_START)),
+         dptr(d(G_SPEC)), dptr(d(G_OFF_SPEC)), dptr(d(cs)), C.c_double(0.8), 36, 12, 3, None)
+    eH, econd, ef = check_histogram_golden_expect()
+    assert np.array_equal(h(tH, H), eH)
+    assert np.array_equal(h(tc, cond), econd)
+    assert np.array_equal(h(tf, f), ef)
+
+
+@pytest.mark.parametrize("cond,cs", [([1] * 9, [1, 1, 1]), ([0, 0, 0, 0, 0, 1, 0, 1, 1], [1, 1, 1]),
+                                     ([1] * 9, [-1, -1, 1])])
+def test_calc_average_golden_gpu(cuda, cond, cs):
+    logG = np.arange(80, dtype=np.float64)
+    c = np.array(cond, np.int8)
+    s = np.array(cs, np.int32)
+    ts = d(np.zeros(20))
+    call("wl_calc_average_log_g", dptr(ts), dptr(d(G_LEN)), dptr(d(logG)), dptr(d(G_START)), dptr(d(G_END)),
+         dptr(d(G_SPEC)), dptr(d(c)), dptr(d(G_OFF_HIST)), dptr(d(G_OFF_SPEC)), dptr(d(G_OFF_SHARED)), dptr(d(s)), 3,
+         4, 3, 80, None)
+    assert np.array_equal(h(ts, np.zeros(20)), calc_average_expect(logG, c, s))
+
+
+def test_redistribute_golden_gpu(cuda):
+    tg = d(np.zeros(80))
+    call("wl_redistribute_g_values", dptr(tg), dptr(d(REDIST_SHARED)), dptr(d(G_LEN)), dptr(d(G_START)),
+         dptr(d(G_END)), dptr(d(np.array([1, 1, 0], np.int32))), dptr(d(G_OFF_HIST)),
+         dptr(d(np.array([1, 1, 1, 1, 1, 0, 1, 0, 0], np.int8))), dptr(d(G_OFF_SHARED)), 3, 4, 3, 80, None)
+    assert np.array_equal(h(tg, np.zeros(80)), REDIST_EXPECT)
+
+
+@pytest.mark.parametrize("case", ["none", "all", "some"])
+def test_reset_d_cond_golden_gpu(cuda, case):
+    beta, S, I, W = 1e-10, 2, 10, 20
+    f = np.full(S * I * W, math.exp(beta))
+    if case == "all":
+        f += 1e-3
+    if case == "some":
+        f[(S - 1) * I * W:] += 1e-3
+    tc = d(np.ones(S * I, np.int8))
+    call("wl_reset_d_cond", dptr(tc), dptr(d(f)), S * I, C.c_double(beta), W, None)
+    c = h(tc, np.ones(1, np.int8))
+    expect = {"none": np.ones(S * I), "all": np.zeros(S * I),
+              "some": np.r_[np.ones((S - 1) * I), np.zeros(I)]}[case]
+    assert np.array_equal(c, expect.astype(np.int8))
+
+
+@pytest.mark.parametrize("case", ["all", "last_not", "none"])
+def test_check_interactions_golden_gpu(cuda, case):
+    S, I = 100, 20
+    cond = np.ones(S * I, np.int8) if case != "none" else np.zeros(S * I, np.int8)
+    if case == "last_not":
+        cond[-1] = 0
+    tcs = d(np.zeros(S, np.int32))
+    call("wl_check_interactions_finished", dptr(tcs), dptr(d(cond)), dptr(d((np.arange(S + 1) * I).astype(np.int32))),
+         I, S, None)
+    cs = h(tcs, np.zeros(1, np.int32))
+    if case == "all":
+        assert (cs == -1).all()
+    elif case == "last_not":
+        assert (cs[:-1] == -1).all() and cs[-1] != -1
+    else:
+        assert (cs != -1).all()
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_single_wl_step_vs_oracle(cuda, boundary):
+    from test_oracle import run_single_step, single_step_setup
+    a = single_step_setup(boundary)
+    b = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in a.items()}
+    run_single_step(a, boundary)
+    t = {k: d(v) for k, v in b.items() if isinstance(v, np.ndarray)}
+    call("wl_wang_landau", dptr(t["lat"]), dptr(t["J"]), dptr(t["H"]), dptr(t["logG"]), dptr(t["f"]), dptr(t["E"]),
+         dptr(t["newE"]), dptr(t["off_iter"]), dptr(t["found"]), dptr(t["start"]), dptr(t["end"]),
+         dptr(t["off_hist"]), dptr(t["off_lat"]), dptr(t["cond"]), dptr(t["spec"]), dptr(t["off_spec"]),
+         dptr(t["cs"]), 4, 4, 1, b["S"] * b["I"] * b["W"], b["I"], b["I"] * b["W"], 42, boundary, None)
+    for k in ("lat", "H", "logG", "E", "off_iter", "found"):
+        assert np.array_equal(h(t[k], a[k]), a[k]), k
+
+
+@pytest.mark.parametrize("even", [1, 0])
+def test_replica_exchange_vs_oracle(cuda, even):
+    from test_oracle import exchange_setup
+    a = exchange_setup()
+    rng = np.random.default_rng(4)
+    a["logG"] = rng.uniform(0, 3, a["logG"].size)  # non-flat: exercise rejections too
+    b = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in a.items()}
+    S, I, W = a["S"], a["I"], a["W"]
+    oracle().wlo_replica_exchange(p(a["off_lat"]), p(a["E"]), p(a["idx"]), p(a["off_iter"]), p(a["start"]),
+                                  p(a["end"]), p(a["off_hist"]), p(a["cs"]), p(a["logG"]), even, 42, I, I * W, S, 0)
+    t = {k: d(v) for k, v in b.items() if isinstance(v, np.ndarray)}
+    call("wl_replica_exchange", dptr(t["off_lat"]), dptr(t["E"]), dptr(t["idx"]), dptr(t["off_iter"]),
+         dptr(t["start"]), dptr(t["end"]), dptr(t["off_hist"]), dptr(t["cs"]), dptr(t["logG"]), even, 42, I, I * W,
+         S, 0, None)
+    for k in ("off_lat", "E", "idx", "off_iter"):
+        assert np.array_equal(h(t[k], a[k]), a[k]), k
+
+
+# ------------------------------------------------------- full plain mainrun
+
+def _c1_state(cls="v", boundary=0, X=4, Y=4, I=4, W=4, S=1, prob=0.1, n=1000, c=20):
+    cfg = engine.RunConfig(X=X, Y=Y, boundary=boundary, num_intervals=I, walker_per_interval=W, overlap=0.25,
+                           alpha=0.8, beta=1e-6, num_iterations=n, exchange_every=c, seed=12345)
+    samples = exact_samples(cfg, S, prob, cls)
+    return engine.build_state(samples, cfg)
+
+
+def _assert_state_equal(a, b, keys=("lattice", "off_lattice", "E", "off_iter", "logG", "H", "factor", "idx",
+                                    "cond", "cond_sample")):
+    for k in keys:
+        x, y = getattr(a, k), getattr(b, k)
+        assert np.array_equal(x, y), f"{k} differs at {np.nonzero(x != y)[0][:10]}"
+
+
+@pytest.mark.parametrize("max_rounds", [1, 7, 60])
+def test_plain_loop_partial_rounds_bit_exact(cuda, max_rounds):
+    st = _c1_state(c=3)
+    ref = st.copy()
+    r = run_loop(ref, max_rounds)
+    ds = engine.DeviceState(st)
+    rc, rounds = engine.run_plain(ds, max_rounds)
+    got = ds.pull()
+    assert rc == 0 and rounds == r
+    _assert_state_equal(got, ref)
+
+
+def test_plain_loop_c1_to_convergence_bit_exact(cuda):
+    st = _c1_state()
+    ref = st.copy()
+    r = run_loop(ref)
+    ds = engine.DeviceState(st)
+    rc, rounds = engine.run_plain(ds)
+    got = ds.pull()
+    assert rc == 0 and rounds == r > 0
+    _assert_state_equal(got, ref)
+    assert (got.cond_sample == -1).all()
+
+
+@pytest.mark.parametrize("boundary,X,Y,S,I,W,cls", [(1, 4, 4, 3, 3, 4, "i"), (0, 5, 5, 2, 5, 3, "h"),
+                                                    (0, 6, 6, 2, 6, 8, "c")])
+def test_plain_loop_multi_sample_bit_exact(cuda, boundary, X, Y, S, I, W, cls):
+    st = _c1_state(cls=cls, boundary=boundary, X=X, Y=Y, I=I, W=W, S=S, n=500, c=5)
+    ref = st.copy()
+    r = run_loop(ref, 400)
+    ds = engine.DeviceState(st)
+    rc, rounds = engine.run_plain(ds, 400)
+    assert rc == 0 and rounds == r
+    _assert_state_equal(ds.pull(), ref)

@@ -1,0 +1,603 @@
+"""Pins the CPU oracle (oracle/wl_oracle.cpp, oracle/exact_dos.cpp).
+
+Parity status per oracle function (what pins it):
+  philox4x32_10        Random123 known-answer vectors (published with the SC'11 paper)
+  exp / log / sqrt     libm within 1-2 ulp on 2e5 random arguments; exp(0)=1, log(e)=1
+  calc_energy          reference goldens test_calc_energy.cu:22,66 (closed form), :115,167 (-6, -4)
+  init_interactions    reference goldens test_init_interactions.cu:48,95,142 (chains), :19 (fraction ~ p)
+  init_lattice         spins are +-1, per-lattice q in (0,1), lattices of a sample differ
+  wang_landau          one-step golden test_wang_landau.cu:162-324 (ln g = 0, f = e => accept, H/ln g +1,
+                       flip matches the recomputed energy); trajectories beyond one step are pinned by the
+                       physics: ln g within tolerance of the exact DOS (transfer matrix) after a full run
+  check_histogram      reference golden test_check_histogram.cu:23-136
+  calc_average_log_g   reference goldens test_calc_average_log_g.cu:39,161,285
+  redistribute_g_values  reference golden test_redistribute_g_values.cu:24-110
+  reset_d_cond         reference goldens test_reset_d_cond.cu:22,50,79
+  check_interactions_finished  reference goldens test_check_interactions.cu:17,58,105
+  init_offsets_histogram  closed form test_init_offsets_histogram.cu:22
+  replica_exchange / fisher_yates  invariants test_replica_exchange.cu:36-283 (flat ln g => every pair
+                       swaps with its shuffled partner; shuffle is a permutation)
+  wang_landau_pre_run  closed forms test_wang_landau_pre_run.cu:695-706,805-811 and the brute-force spectrum
+                       of integration_prerun_matching_hist_energies.py:65-97
+  interval_results     the reference's own src/interval.cpp (oracle/_ref) and golden test_interval.cpp:12-60
+  stitch               invariants integration_mainrun.py:118 (LSE = N ln 2) and energy set == spectrum;
+                       ln g vs the exact DOS after a full run
+  exact DOS (transfer) brute-force enumeration on small lattices, sum g = 2^N, g(E) = g(-E) when bipartite
+  log_partition        closed form on the exact DOS (Z(0) = 2^N) and direct summation
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from helpers import exact_samples, oracle_disorder, oracle_prerun
+from oracle_lib import exact_lng, oracle, p, ref, run_loop
+from qec_pf_wang_landau_amd import engine
+
+
+def u32(a):
+    return np.array(a, np.uint32)
+
+
+# ----------------------------------------------------------------- RNG / math
+
+@pytest.mark.parametrize("ctr,key,expect", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_known_answers(oracle, ctr, key, expect):
+    out = np.zeros(4, np.uint32)
+    oracle.wlo_philox4x32_10(p(u32(ctr)), p(u32(key)), p(out))
+    assert out.tolist() == expect
+
+
+def test_uniform_is_open_unit_interval_and_uses_all_words(oracle):
+    us = [oracle.wlo_uniform(7, 0, 3, d) for d in range(4000)]
+    assert 0.0 < min(us) and max(us) < 1.0
+    assert abs(np.mean(us) - 0.5) < 0.02
+    # draws 0..3 are the four words of one Philox block
+    out = np.zeros(4, np.uint32)
+    oracle.wlo_philox4x32_10(p(u32([0, 0, 3, 0])), p(u32([7, 0])), p(out))
+    assert [oracle.wlo_uniform(7, 0, 3, d) for d in range(4)] == [(float(x) + 0.5) / 2**32 for x in out]
+
+
+def test_pick_matches_reference_formula(oracle):
+    for u in [1e-9, 0.3, 0.5, 0.999999999]:
+        for n in [2, 16, 36, 1024]:
+            assert oracle.wlo_pick(u, n) == int(u * ((n - 1) + 0.999999))
+    assert oracle.wlo_pick(1.0 - 2**-33, 16) == 15
+
+
+def _ulp_err(a, b):
+    return abs(a - b) / math.ulp(b)
+
+
+@pytest.mark.parametrize("name,ref_fn,lo,hi,maxulp", [
+    ("wlo_exp", math.exp, -740.0, 709.0, 1.0),
+    ("wlo_log", math.log, None, None, 2.0),
+    ("wlo_sqrt", math.sqrt, None, None, 1.0),
+])
+def test_math_against_libm(oracle, name, ref_fn, lo, hi, maxulp):
+    rng = np.random.default_rng(5)
+    xs = rng.uniform(lo, hi, 20000) if lo is not None else np.exp(rng.uniform(-700, 700, 20000))
+    fn = getattr(oracle, name)
+    worst = max(_ulp_err(fn(float(x)), ref_fn(float(x))) for x in xs)
+    assert worst <= maxulp
+
+
+def test_math_special_values(oracle):
+    assert oracle.wlo_exp(0.0) == 1.0
+    assert oracle.wlo_exp(-800.0) == 0.0
+    assert abs(oracle.wlo_log(math.e) - 1.0) < 1e-15
+    assert oracle.wlo_log(1.0) == 0.0
+    assert oracle.wlo_sqrt(4.0) == 2.0
+    # the flatness golden compares f = sqrt(e) directly
+    assert oracle.wlo_sqrt(math.e) == math.sqrt(math.e)
+    # reset_d_cond golden: factor = exp(beta) must not exceed the oracle's exp(beta)
+    assert oracle.wlo_exp(1e-10) == math.exp(1e-10)
+
+
+# ------------------------------------------------------------------ energies
+
+def test_calc_energy_closed_forms(oracle):
+    for boundary, expect in [(0, -2 * 16), (1, -2 * 16 + 4 + 4)]:
+        lat = np.ones(16, np.int8)
+        J = np.ones(32, np.int8)
+        off = np.zeros(1, np.int32)
+        E = np.zeros(1, np.int32)
+        oracle.wlo_calc_energy(boundary, p(E), p(lat), p(J), p(off), 4, 4, 1, 1)
+        assert E[0] == expect
+
+
+GOLDEN_LATTICE = np.array([1, 1, -1, 1, -1, -1, 1, -1, 1, -1, 1, -1, -1, -1, -1, -1], np.int8)
+GOLDEN_J = np.array([1, -1, 1, 1, 1, 1, 1, 1, -1, 1, 1, -1, 1, 1, 1, 1,
+                     1, 1, -1, 1, 1, 1, 1, 1, -1, 1, 1, 1, 1, 1, 1, 1], np.int8)
+
+
+@pytest.mark.parametrize("boundary,expect", [(0, -6), (1, -4)])
+def test_calc_energy_known_answer(oracle, boundary, expect):
+    off = np.zeros(1, np.int32)
+    E = np.zeros(1, np.int32)
+    oracle.wlo_calc_energy(boundary, p(E), p(GOLDEN_LATTICE), p(GOLDEN_J), p(off), 4, 4, 1, 1)
+    assert E[0] == expect
+    assert engine.lattice_energies(GOLDEN_LATTICE, GOLDEN_J, 4, 4, boundary)[0] == expect
+
+
+# ------------------------------------------------------------------ disorder
+
+def _golden_chain(cls):
+    a = np.ones((2, 8, 4), np.int8)
+    if cls in ("h", "c"):
+        a[:, 4, :] = -1
+    if cls in ("v", "c"):
+        a[:, 0:4, 0] = -1
+    return a.reshape(-1)
+
+
+@pytest.mark.parametrize("cls", ["h", "v", "c"])
+def test_error_chains_golden(cls):
+    J = oracle_disorder(1, 2, 4, 4, 0.0, cls)
+    assert np.array_equal(J, _golden_chain(cls))
+
+
+def test_golden_chain_matches_reference_vectors():
+    # test_init_interactions.cu:62-77 ('h'): row 4 of each sample's 8x4 block flipped
+    h = _golden_chain("h").reshape(2, 8, 4)
+    assert (h[:, 4] == -1).all() and (h[:, :4] == 1).all() and (h[:, 5:] == 1).all()
+    # :109-124 ('v'): first entry of rows 0..3 flipped
+    v = _golden_chain("v").reshape(2, 8, 4)
+    assert (v[:, :4, 0] == -1).all() and (v[:, :4, 1:] == 1).all() and (v[:, 4:] == 1).all()
+
+
+def test_disorder_fraction_close_to_p():
+    J = oracle_disorder(123, 1000, 10, 10, 0.15, "i")
+    frac = np.mean(J == -1)
+    assert abs(frac - 0.15) / 0.15 < 0.01
+
+
+def test_init_lattice(oracle):
+    X = Y = 6
+    lat = np.zeros(8 * 36, np.int8)
+    probs = np.zeros(8, np.float32)
+    oracle.wlo_init_lattice(p(lat), p(probs), X, Y, 8, 99, 4)
+    assert set(np.unique(lat).tolist()) <= {-1, 1}
+    assert ((probs > 0) & (probs < 1)).all()
+    l = lat.reshape(8, 36)
+    assert len({bytes(r) for r in l}) == 8
+
+
+# ------------------------------------------------- kernel goldens (reference)
+
+# Shared fixture of test_check_histogram.cu / test_calc_average_log_g.cu / test_redistribute_g_values.cu
+G_LEN = np.array([7 * 4, 9 * 4, 4 * 4], np.int32)
+G_START = np.array([-2, -1, 0, -4, -2, 0, -2, -1, 0], np.int32)
+G_END = np.array([-1, 0, 2, -2, 0, 2, -2, -1, 1], np.int32)
+G_OFF_HIST = np.array([
+    0, 2, 2 * 2, 3 * 2, 4 * 2, 5 * 2, 6 * 2, 7 * 2,
+    8 * 2, 8 * 2 + 3, 8 * 2 + 2 * 3, 8 * 2 + 3 * 3, 8 * 2 + 4 * 3, 8 * 2 + 5 * 3, 8 * 2 + 6 * 3, 8 * 2 + 7 * 3,
+    8 * 2 + 8 * 3, 8 * 2 + 9 * 3, 8 * 2 + 10 * 3, 8 * 2 + 11 * 3, 8 * 2 + 12 * 3, 8 * 2 + 13 * 3, 8 * 2 + 14 * 3,
+    8 * 2 + 15 * 3, 8 * 2 + 16 * 3, 8 * 2 + 16 * 3 + 1, 8 * 2 + 16 * 3 + 2, 8 * 2 + 16 * 3 + 3,
+    8 * 2 + 16 * 3 + 4, 8 * 2 + 16 * 3 + 5, 8 * 2 + 16 * 3 + 6, 8 * 2 + 16 * 3 + 7,
+    8 * 2 + 16 * 3 + 8, 8 * 2 + 16 * 3 + 8 + 2, 8 * 2 + 16 * 3 + 8 + 2 * 2, 8 * 2 + 16 * 3 + 8 + 3 * 2], np.int32)
+G_OFF_SPEC = np.array([0, 7, 16], np.int32)
+G_SPEC = np.ones(20, np.int8)
+G_OFF_SHARED = np.array([0, 2, 4, 7, 10, 13, 16, 17, 18], np.int64)
+
+
+def check_histogram_golden_inputs():
+    H = np.full(80, 10, np.uint64)
+    H[0] = 1
+    cond = np.array([0, 0, 0, 0, 0, 0, 0, 0, 1], np.int8)
+    cs = np.array([1, 1, 1], np.int32)
+    f = np.full(36, math.e)
+    return H, f, cond, cs
+
+
+def check_histogram_golden_expect():
+    H = np.zeros(80, np.uint64)
+    idx = np.arange(80)
+    H[(idx < 8) | (idx > 71)] = 10
+    H[0] = 1
+    cond = np.array([0, 1, 1, 1, 1, 1, 1, 1, 1], np.int8)
+    f = np.full(36, math.sqrt(math.e))
+    f[(np.arange(36) // 4 == 0) | (np.arange(36) // 4 == 8)] = math.e
+    return H, cond, f
+
+
+def test_check_histogram_golden(oracle):
+    H, f, cond, cs = check_histogram_golden_inputs()
+    oracle.wlo_check_histogram(p(H), p(f), p(cond), p(G_OFF_HIST), p(G_END), p(G_START), p(G_SPEC), p(G_OFF_SPEC),
+                               p(cs), 0.8, 36, 12, 3)
+    eH, econd, ef = check_histogram_golden_expect()
+    assert np.array_equal(H, eH)
+    assert np.array_equal(cond, econd)
+    assert np.array_equal(f, ef)
+
+
+def calc_average_expect(logG, cond, cs):
+    """The CPU restatement inside test_calc_average_log_g.cu:124-140."""
+    exp = np.zeros(20)
+    offs = list(G_OFF_HIST)
+    for i in range(80):
+        import bisect
+        idx = bisect.bisect_left(offs, i)
+        w = idx - 1 if (idx != 0 and (idx >= len(offs) or offs[idx] != i)) else idx
+        k = w // 4
+        e = i - offs[w]
+        if cond[k] == 1 and cs[w // 12] == 1:
+            exp[G_OFF_SHARED[k] + e] += logG[i] / 4.0
+    return exp
+
+
+@pytest.mark.parametrize("cond,cs", [
+    ([1] * 9, [1, 1, 1]),
+    ([0, 0, 0, 0, 0, 1, 0, 1, 1], [1, 1, 1]),
+    ([1] * 9, [-1, -1, 1]),
+])
+def test_calc_average_log_g_golden(oracle, cond, cs):
+    logG = np.arange(80, dtype=np.float64)
+    shared = np.zeros(20)
+    c = np.array(cond, np.int8)
+    s = np.array(cs, np.int32)
+    oracle.wlo_calc_average_log_g(p(shared), p(G_LEN), p(logG), p(G_START), p(G_END), p(G_SPEC), p(c), p(G_OFF_HIST),
+                                  p(G_OFF_SPEC), p(G_OFF_SHARED), p(s), 3, 4, 3, 80)
+    assert np.array_equal(shared, calc_average_expect(logG, c, s))
+
+
+REDIST_SHARED = np.array([1.1, 2.2, 3.3, 4.4, 5.5, 6.6, 7.7, 8.8, 9.9, 10.10, 11.11, 12.12, 13.13, 14.14, 15.15,
+                          16.16, 17.17, 18.18, 19.19, 20.20])
+REDIST_EXPECT = np.array([
+    1.1, 2.2, 1.1, 2.2, 1.1, 2.2, 1.1, 2.2, 3.3, 4.4, 3.3, 4.4, 3.3, 4.4, 3.3, 4.4, 5.5, 6.6, 7.7, 5.5,
+    6.6, 7.7, 5.5, 6.6, 7.7, 5.5, 6.6, 7.7, 8.8, 9.9, 10.10, 8.8, 9.9, 10.10, 8.8, 9.9, 10.10, 8.8,
+    9.9, 10.10, 11.11, 12.12, 13.13, 11.11, 12.12, 13.13, 11.11, 12.12, 13.13, 11.11, 12.12, 13.13, 0, 0, 0, 0,
+    0, 0, 0, 0, 0, 0, 0, 0, 17.17, 17.17, 17.17, 17.17, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0])
+
+
+def test_redistribute_golden(oracle):
+    logG = np.zeros(80)
+    cond = np.array([1, 1, 1, 1, 1, 0, 1, 0, 0], np.int8)
+    cs = np.array([1, 1, 0], np.int32)
+    oracle.wlo_redistribute_g_values(p(logG), p(REDIST_SHARED), p(G_LEN), p(G_START), p(G_END), p(cs),
+                                     p(G_OFF_HIST), p(cond), p(G_OFF_SHARED), 3, 4, 3, 80)
+    assert np.array_equal(logG, REDIST_EXPECT)
+
+
+@pytest.mark.parametrize("case", ["none", "all", "some"])
+def test_reset_d_cond_golden(oracle, case):
+    beta, S, I, W = 1e-10, 2, 10, 20
+    cond = np.ones(S * I, np.int8)
+    f = np.full(S * I * W, math.exp(beta))
+    if case == "all":
+        f += 1e-3
+    if case == "some":
+        f[(S - 1) * I * W:] += 1e-3
+    oracle.wlo_reset_d_cond(p(cond), p(f), S * I, beta, W)
+    if case == "none":
+        assert (cond == 1).all()
+    elif case == "all":
+        assert (cond == 0).all()
+    else:
+        assert (cond[: (S - 1) * I] == 1).all() and (cond[(S - 1) * I:] == 0).all()
+
+
+@pytest.mark.parametrize("case", ["all", "last_not", "none"])
+def test_check_interactions_golden(oracle, case):
+    S, I = 100, 20
+    cond = np.ones(S * I, np.int8) if case != "none" else np.zeros(S * I, np.int8)
+    if case == "last_not":
+        cond[-1] = 0
+    cs = np.zeros(S, np.int32)
+    oracle.wlo_check_interactions_finished(p(cs), p(cond), I, S)
+    if case == "all":
+        assert (cs == -1).all()
+    elif case == "last_not":
+        assert (cs[:-1] == -1).all() and cs[-1] != -1
+    else:
+        assert (cs != -1).all()
+
+
+def test_init_offsets_histogram_closed_form(oracle):
+    rng = np.random.default_rng(3)
+    S, I, W = 10, 10, 5
+    starts, ends, lens = [], [], []
+    for _ in range(S):
+        iv = engine.interval_results(int(rng.integers(-200, -190)), int(rng.integers(190, 200)), I, W, 0.25)
+        starts.append(iv.start)
+        ends.append(iv.end)
+        lens.append(iv.len_histogram_over_all_walkers)
+    st, en = np.concatenate(starts), np.concatenate(ends)
+    ln = np.array(lens, np.int32)
+    off = np.zeros(S * I * W, np.int32)
+    oracle.wlo_init_offsets_histogram(p(off), p(st), p(en), p(ln), I, S * I * W, W)
+    expect = []
+    base = 0
+    for s in range(S):
+        len0 = en[s * I] - st[s * I] + 1
+        for k in range(I):
+            lenk = en[s * I + k] - st[s * I + k] + 1
+            expect += [base + k * W * len0 + w * lenk for w in range(W)]
+        base += ln[s]
+    assert off.tolist() == expect
+
+
+# ------------------------------------------------------------- wang-landau
+
+def single_step_setup(boundary, seed=42):
+    """test_wang_landau.cu:162-232: S=10, I=10, W=5 on 4x4, intervals [-2N, 2N],
+    ln g = 0, f = e, every spectrum flag set."""
+    S, I, W, X, Y = 10, 10, 5, 4, 4
+    N = X * Y
+    rng = np.random.default_rng(seed)
+    lat = rng.choice(np.array([-1, 1], np.int8), S * I * W * N)
+    J = rng.choice(np.array([-1, 1], np.int8), S * 2 * N)
+    off_lat = (np.arange(S * I * W) * N).astype(np.int32)
+    E = np.zeros(S * I * W, np.int32)
+    oracle().wlo_calc_energy(boundary, p(E), p(lat), p(J), p(off_lat), X, Y, S * I * W, I * W)
+    L = 4 * N + 1
+    start = np.full(S * I, -2 * N, np.int32)
+    end = np.full(S * I, 2 * N, np.int32)
+    off_hist = (np.arange(S * I * W) * L).astype(np.int32)
+    return dict(S=S, I=I, W=W, X=X, Y=Y, N=N, lat=lat, J=J, off_lat=off_lat, E=E, start=start, end=end,
+                off_hist=off_hist, H=np.zeros(S * I * W * L, np.uint64), logG=np.zeros(S * I * W * L),
+                f=np.full(S * I * W, math.e), off_iter=np.zeros(S * I * W, np.uint64),
+                newE=np.zeros(S * I * W, np.int32), found=np.zeros(S * I * W, np.int32),
+                cond=np.zeros(S * I, np.int8), spec=np.ones(S * (2 * N) + 1 + 2 * N, np.int8),
+                off_spec=np.zeros(S, np.int32), cs=np.zeros(S, np.int32), L=L)
+
+
+def run_single_step(d, boundary, seed=42):
+    oracle().wlo_wang_landau(p(d["lat"]), p(d["J"]), p(d["H"]), p(d["logG"]), p(d["f"]), p(d["E"]), p(d["newE"]),
+                             p(d["off_iter"]), p(d["found"]), p(d["start"]), p(d["end"]), p(d["off_hist"]),
+                             p(d["off_lat"]), p(d["cond"]), p(d["spec"]), p(d["off_spec"]), p(d["cs"]), d["X"], d["Y"],
+                             1, d["S"] * d["I"] * d["W"], d["I"], d["I"] * d["W"], seed, boundary)
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_single_wl_step_golden(boundary):
+    d = single_step_setup(boundary)
+    lat0, E0 = d["lat"].copy(), d["E"].copy()
+    run_single_step(d, boundary)
+    N, L = d["N"], d["L"]
+    nW = d["S"] * d["I"] * d["W"]
+    flips = (d["lat"] != lat0).reshape(nW, N).sum(axis=1)
+    assert (flips == 1).all(), "every proposal is accepted when ln g is flat"
+    recomputed = np.concatenate([
+        engine.lattice_energies(d["lat"][t * N:(t + 1) * N], d["J"][(t // 50) * 2 * N:(t // 50 + 1) * 2 * N], 4, 4,
+                                boundary) for t in range(nW)])
+    assert np.array_equal(recomputed, d["E"])
+    expH = np.zeros(nW * L, np.uint64)
+    expG = np.zeros(nW * L)
+    for t in range(nW):
+        expH[t * L + d["E"][t] + 2 * N] = 1
+        expG[t * L + d["E"][t] + 2 * N] = 1.0
+    assert np.array_equal(d["H"], expH)
+    assert np.max(np.abs(d["logG"] - expG)) < 1e-15
+    assert (d["off_iter"] == 2).all()
+    assert (d["E"] != E0).any()
+
+
+# ---------------------------------------------------------- replica exchange
+
+def exchange_setup():
+    """test_replica_exchange.cu:36-80: flat ln g, every energy inside both neighbours' ranges."""
+    S, I, W = 10, 10, 5
+    start = np.zeros(S * I, np.int32)
+    end = np.zeros(S * I, np.int32)
+    E = np.zeros(S * I * W, np.int32)
+    off_hist = np.zeros(S * I * W, np.int32)
+    off_lat = np.zeros(S * I * W, np.int32)
+    inc = 0
+    for i in range(S * I):
+        iid, sid = i % I, i // I
+        start[i] = iid + sid
+        end[i] = iid + sid + 4
+        for j in range(W):
+            E[i * W + j] = iid + sid + 2
+            off_hist[i * W + j] = inc
+            inc += 5
+            off_lat[i * W + j] = (i * W + j) * 9
+    return dict(S=S, I=I, W=W, start=start, end=end, E=E, off_hist=off_hist, off_lat=off_lat,
+                logG=np.ones(inc), cs=np.ones(S, np.int32), idx=np.tile(np.arange(W, dtype=np.int32), S * I),
+                off_iter=np.zeros(S * I * W, np.uint64))
+
+
+def expected_permutations(seed, S, I, W, oracle_lib_oracle):
+    """The shuffle thread 0 of each interval performs, from counter 0."""
+    perms = []
+    for k in range(S * I):
+        idx = list(range(W))
+        t0 = k * W
+        for d, i in enumerate(range(W - 1, 0, -1)):
+            u = oracle_lib_oracle.wlo_uniform(seed + t0 // (I * W), 0, t0 % (I * W), d)
+            r = oracle_lib_oracle.wlo_pick(u, i + 1)
+            idx[i], idx[r] = idx[r], idx[i]
+        perms.append(idx)
+    return perms
+
+
+@pytest.mark.parametrize("even", [1, 0])
+def test_replica_exchange_flat_log_g_swaps_every_pair(oracle, even):
+    d = exchange_setup()
+    S, I, W = d["S"], d["I"], d["W"]
+    perms = expected_permutations(42, S, I, W, oracle)
+    for perm in perms:
+        assert sorted(perm) == list(range(W)) and sum(perm) == W * (W - 1) // 2
+    E0, L0 = d["E"].copy(), d["off_lat"].copy()
+    oracle.wlo_replica_exchange(p(d["off_lat"]), p(d["E"]), p(d["idx"]), p(d["off_iter"]), p(d["start"]),
+                                p(d["end"]), p(d["off_hist"]), p(d["cs"]), p(d["logG"]), even, 42, I, I * W, S, 0)
+    for i in range(S * I):
+        iid = i % I
+        active = iid < I - 1 and (iid % 2 == 0) == bool(even)
+        for j in range(W):
+            t = i * W + j
+            if active:
+                sw = (i + 1) * W + perms[i][j]
+                assert d["off_lat"][t] == L0[sw] and d["off_lat"][sw] == L0[t]
+                assert d["E"][t] == E0[sw] and d["E"][sw] == E0[t]
+                expect_draws = (W - 1 if j == 0 else 0) + 1
+                assert d["off_iter"][t] == expect_draws
+            elif not (iid > 0 and (iid - 1) % 2 == (0 if even else 1)):
+                assert d["off_lat"][t] == L0[t]
+
+
+# ------------------------------------------------------------------- prerun
+
+def _prerun_spectrum(X, Y, boundary, J, walkers, iters, seed=5):
+    pre = oracle_prerun(J, X, Y, boundary, seed, 1, walkers, iters, 5)
+    h, fl, fe = pre[0]
+    return set((np.nonzero(h)[0] - 2 * X * Y).tolist()), fl, fe
+
+
+def test_prerun_periodic_ferromagnet_closed_form():
+    J = np.ones(32, np.int8)
+    energies, fl, fe = _prerun_spectrum(4, 4, 0, J, 100, 3000)
+    expect = {e for e in range(-32, 33) if e % 4 == 0 and abs(e) != 28}
+    assert energies == expect
+    # stored lattices lie in their prerun interval and have the stated energies
+    iv = engine.interval_results(-32, 32, 5, 1, 1.0)
+    for lat, e in zip(fl, fe):
+        k = min((e + 32) // iv.len_interval, 4)
+        assert iv.start[k] <= e <= iv.end[k]
+
+
+def test_prerun_open_ferromagnet_closed_form():
+    J = np.ones(32, np.int8)
+    energies, _, _ = _prerun_spectrum(4, 4, 1, J, 100, 3000)
+    expect = {e for e in range(-24, 25) if e % 2 == 0 and abs(e) != 22}
+    assert energies == expect
+
+
+@pytest.mark.parametrize("boundary,prob", [(0, 0.0), (0, 1.0), (1, 0.0), (0, 0.3), (1, 0.3)])
+def test_prerun_spectrum_equals_bruteforce(oracle, boundary, prob):
+    X = Y = 4
+    J = oracle_disorder(11, 1, X, Y, prob, "i")
+    energies, _, _ = _prerun_spectrum(X, Y, boundary, J, 64, 4000)
+    counts = np.zeros(4 * 16 + 1, np.uint64)
+    oracle.wlo_exact_dos_bruteforce(p(J), X, Y, boundary, p(counts))
+    assert energies == set((np.nonzero(counts)[0] - 32).tolist())
+
+
+# ---------------------------------------------------------------- intervals
+
+def _oracle_intervals(e_min, e_max, I, W, o):
+    st = np.zeros(I, np.int32)
+    en = np.zeros(I, np.int32)
+    tot = np.zeros(1, np.int64)
+    ln = np.zeros(1, np.int32)
+    rc = oracle().wlo_interval_results(e_min, e_max, I, W, o, p(st), p(en), p(tot), p(ln))
+    return rc, st, en, int(tot[0]), int(ln[0])
+
+
+def test_interval_golden():
+    rc, st, en, tot, ln = _oracle_intervals(-128, 128, 16, 8, 0.25)
+    assert rc == 0 and ln == 54 and st[0] == -128 and st[15] == 67 and en[15] == 128
+    assert tot == 62 * 8 + 54 * 8 * 15
+
+
+@pytest.mark.parametrize("args", [(-16, 16, 64, 8, 1.0), (-16, 16, 64, 8, 1.1)])
+def test_interval_errors(args):
+    assert _oracle_intervals(*args)[0] != 0
+
+
+def test_interval_matches_reference_implementation():
+    r = ref()
+    if r is None:
+        pytest.skip("oracle/_ref was not built (reference sources absent)")
+    rng = np.random.default_rng(0)
+    for _ in range(500):
+        e_min = int(rng.integers(-2000, 0))
+        e_max = e_min + int(rng.integers(10, 4000))
+        I = int(rng.integers(1, 65))
+        o = float(np.float32(rng.choice([0.0, 0.1, 0.25, 0.5, 0.75, 1.0, rng.uniform(0, 1)])))
+        rc, st, en, tot, ln = _oracle_intervals(e_min, e_max, I, 8, o)
+        st2 = np.zeros(I, np.int32)
+        en2 = np.zeros(I, np.int32)
+        tot2 = np.zeros(1, np.int64)
+        ln2 = np.zeros(1, np.int32)
+        rc2 = r.wlref_interval_results(e_min, e_max, I, 8, o, p(st2), p(en2), p(tot2), p(ln2))
+        assert (rc == 0) == (rc2 == 0)
+        if rc == 0:
+            assert np.array_equal(st, st2) and np.array_equal(en, en2) and tot == tot2[0] and ln == ln2[0]
+
+
+# ------------------------------------------------------------- exact DOS
+
+@pytest.mark.parametrize("X,Y,boundary,prob", [(3, 3, 0, 0.3), (4, 4, 0, 0.1), (3, 5, 0, 0.5), (4, 3, 1, 0.2),
+                                                (4, 4, 1, 0.4), (2, 4, 0, 0.5), (5, 4, 0, 0.2)])
+def test_transfer_matrix_matches_bruteforce(oracle, X, Y, boundary, prob):
+    J = oracle_disorder(3, 1, X, Y, prob, "c")
+    N = X * Y
+    bf = np.zeros(4 * N + 1, np.uint64)
+    oracle.wlo_exact_dos_bruteforce(p(J), X, Y, boundary, p(bf))
+    lng = np.zeros(4 * N + 1)
+    cnt = np.zeros(4 * N + 1, np.uint64)
+    assert oracle.wlo_exact_dos_transfer(p(J), X, Y, boundary, p(lng), p(cnt)) == 0
+    assert np.array_equal(cnt, bf)
+    assert int(bf.sum()) == 2 ** N
+
+
+def test_exact_dos_ferromagnet_symmetry_and_ground_state():
+    J = np.ones(72, np.int8)
+    lng = exact_lng(J, 6, 6, 0)
+    finite = np.isfinite(lng)
+    assert np.array_equal(finite, finite[::-1])
+    assert np.allclose(lng[finite], lng[::-1][finite])
+    assert abs(lng[0] - math.log(2)) < 1e-12  # two ground states at E = -72
+    assert abs(np.log(np.sum(np.exp(lng[finite] - lng[finite].max()))) + lng[finite].max() - 36 * math.log(2)) < 1e-9
+
+
+def test_log_partition_closed_form(oracle):
+    J = oracle_disorder(8, 1, 4, 4, 0.2, "i")
+    lng = exact_lng(J, 4, 4, 0)
+    E = np.nonzero(np.isfinite(lng))[0].astype(np.int32) - 32
+    g = np.ascontiguousarray(lng[np.isfinite(lng)])
+    assert abs(oracle.wlo_log_partition(p(E), p(g), E.size, 0.0) - 16 * math.log(2)) < 1e-12
+    beta = 0.4
+    direct = math.log(sum(math.exp(gi - beta * ei) for gi, ei in zip(g, E)))
+    assert abs(oracle.wlo_log_partition(p(E), p(g), E.size, beta) - direct) < 1e-12
+    assert abs(oracle.wlo_nishimori_beta(0.1) - 0.5 * math.log(9.0)) < 1e-15
+
+
+# -------------------------------------------------- full mainrun (physics)
+
+def _run_c1(cls="v", boundary=0, beta=1e-6):
+    cfg = engine.RunConfig(X=4, Y=4, boundary=boundary, num_intervals=4, walker_per_interval=4, overlap=0.25,
+                           alpha=0.8, beta=beta, num_iterations=1000, exchange_every=20, seed=12345)
+    samples = exact_samples(cfg, 1, 0.1, cls)
+    st = engine.build_state(samples, cfg)
+    r = run_loop(st)
+    return cfg, samples, st, r
+
+
+def _stitch_oracle(st, s=0):
+    cfg = st.cfg
+    start, end = st.sample_intervals(s)
+    g = np.ascontiguousarray(st.sample_logG(s))
+    cap = int(end[-1] - start[0] + 1)
+    oe = np.zeros(cap, np.int32)
+    og = np.zeros(cap)
+    n = oracle().wlo_stitch(p(g), p(np.ascontiguousarray(start)), p(np.ascontiguousarray(end)), cfg.num_intervals,
+                            cfg.walker_per_interval, cfg.X, cfg.Y, p(oe), p(og), cap)
+    assert n > 0
+    return oe[:n], og[:n]
+
+
+def test_mainrun_c1_converges_to_exact_dos():
+    cfg, samples, st, rounds = _run_c1()
+    assert rounds > 0
+    assert (st.cond_sample == -1).all()
+    assert (st.factor <= math.exp(1e-6) + 1e-10).all()
+    E, lng = _stitch_oracle(st)
+    # integration_mainrun.py:118: LSE(ln g) = N ln 2; energy set = spectrum
+    m = lng.max()
+    assert abs(m + math.log(np.sum(np.exp(lng - m))) - 16 * math.log(2)) < 1e-12
+    spec_E = set((np.nonzero(samples[0].mask)[0] + samples[0].e_min).tolist())
+    assert set(E.tolist()) == spec_E
+    exact = exact_lng(samples[0].J, 4, 4, 0)[E + 32]
+    err = np.max(np.abs(lng - exact))
+    assert err < 0.05, err
