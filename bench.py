@@ -1,0 +1,211 @@
+"""Benchmark of the replica-exchange Wang-Landau hot path on one MI355X.
+
+Workload (default): 6x6 periodic random-bond Ising model at p = 0.04, 100
+disorder samples from seed 12345, each with all four error classes (i/v/h/c)
+-> 400 tasks; 10 intervals with overlap 0.25, 8 walkers per interval,
+flatness 0.8, ln f from 1 down to 1e-6, replica exchange every 20 rounds,
+1000 WL attempts per walker per round.  Disorder and prerun (spectrum and seed
+lattices) are computed on the GPU before timing; the state is resident in HBM
+when the timed region starts.
+
+One step = the whole 400-task batch run to ln f <= 1e-6 by the batched
+persistent kernel (one launch).  The plain launch sequence (the reference's
+mainrun loop, one run per error class over the 100 samples) is timed on the
+same inputs once, and the CPU oracle (OpenMP over tasks) on a bounded sample.
+
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+BYTES_PER_ATTEMPT = 34  # ln g reads at E_new and E_cur, ln g write, int32 H read+write, spin/coupling bits
+HBM_PEAK_GBS = 8000.0
+
+METRIC = "spin-flip attempts/s + samples/s to ln f=1e-6 (batched vs plain); p99 latency"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--samples", type=int, default=100)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-tasks", type=int, default=16)
+    return ap.parse_args()
+
+
+def build_batch(engine, samples: int):
+    cfg = engine.RunConfig(X=6, Y=6, boundary=0, num_intervals=10, walker_per_interval=8, overlap=0.25, alpha=0.8,
+                           beta=1e-6, num_iterations=1000, exchange_every=20, seed=12345)
+    per_class = {}
+    for cls in engine.ERROR_CLASSES:
+        per_class[cls] = engine.problem_gpu(cfg, samples, 0.04, cls, prerun_walkers=128, prerun_iters=10000,
+                                            prerun_intervals=16)
+    states = [engine.build_state(per_class[c], cfg) for c in engine.ERROR_CLASSES]
+    return cfg, per_class, states
+
+
+def attempts_of(batch_arrays, tasks) -> int:
+    rounds = batch_arrays["rounds"].astype(np.int64)
+    walkers = (tasks["I"] * tasks["W"]).astype(np.int64)
+    return int(np.sum(rounds * walkers * tasks["num_iterations"].astype(np.int64)))
+
+
+def cpu_baseline(engine, per_class, cfg, n_tasks: int):
+    """Oracle (OpenMP over tasks) on the first n_tasks/4 samples of every class."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import oracle, p  # noqa: PLC0415
+    o = oracle()
+    k = max(1, n_tasks // 4)
+    samples = []
+    for c in engine.ERROR_CLASSES:
+        samples += per_class[c][:k]
+    st = engine.build_state(samples, cfg)
+    rounds = np.zeros(st.S, np.int64)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    t0 = time.perf_counter()
+    failed = o.wlo_run_tasks(st.S, p(st.lattice), p(st.J), p(st.H), p(st.logG), p(st.factor), p(st.E),
+                             p(st.off_iter), p(st.off_lattice), p(st.idx), p(st.cond), p(st.cond_sample),
+                             p(st.start), p(st.end), p(st.off_hist), p(st.spectrum), p(st.off_spec), p(st.len_hist),
+                             p(st.keys), cfg.num_intervals, cfg.walker_per_interval, cfg.X, cfg.Y, cfg.boundary,
+                             cfg.num_iterations, cfg.alpha, cfg.beta, cfg.exchange_every, 1 << 40, p(rounds), threads)
+    dt = time.perf_counter() - t0
+    attempts = int(rounds.sum()) * cfg.num_intervals * cfg.walker_per_interval * cfg.num_iterations
+    return {
+        "value": k / dt,
+        "unit": "samples/s",
+        "cores": int(o.wlo_max_threads()),
+        "kind": "port",
+        "sample": f"{k} samples x 4 classes ({st.S} tasks) of the same workload run to ln f<=1e-6 by the "
+                  f"oracle's single-threaded transliteration, OpenMP over tasks; {dt:.1f} s, failed={failed}",
+        "attempts_per_s": attempts / dt,
+        "seconds": dt,
+        "cpu": platform.processor() or platform.machine(),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the batched form shards by task "
+              "across GPUs with no collectives, which this build does not launch.", file=sys.stderr)
+        sys.exit(2)
+    import torch  # noqa: PLC0415
+
+    from qec_pf_wang_landau_amd import engine  # noqa: PLC0415
+
+    torch.cuda.set_device(0)
+    t_setup = time.perf_counter()
+    cfg, per_class, states = build_batch(engine, args.samples)
+    batch = engine.merge_states(states)
+    dev = engine.BatchDevice(batch)
+    launcher = engine.FusedLauncher(dev)
+    setup_s = time.perf_counter() - t_setup
+    print(f"[bench] setup {setup_s:.1f}s: {batch.tasks.size} tasks, grid {launcher.grid} x {launcher.block}, "
+          f"LDS {launcher.lds_bytes} B", file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        launcher.restore()
+        launcher.launch()
+        torch.cuda.synchronize()
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+
+    ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        launcher.restore()
+        ev0[k].record()
+        launcher.launch()
+        ev1[k].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in zip(ev0, ev1)]
+    result = dev.pull()
+    status = result.arrays["status"]
+    n_ok = int(np.sum(status == 1))
+    attempts = attempts_of(result.arrays, batch.tasks)
+    step_s = wall / args.steps
+    kern_s = float(np.mean(kernel_ms)) / 1e3
+    samples_per_s = args.samples / step_s
+    print(f"[bench] fused: {step_s * 1e3:.1f} ms/step (kernel {kern_s * 1e3:.1f} ms), {attempts:.3e} attempts, "
+          f"{n_ok}/{status.size} tasks finished", file=sys.stderr, flush=True)
+
+    plain = None
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        plain_rounds = 0
+        for st in states:
+            ds = engine.DeviceState(st.copy())
+            torch.cuda.synchronize()
+            rc, r = engine.run_plain(ds)
+            plain_rounds += r
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t0
+        plain = {"seconds": plain_s, "samples_per_s": args.samples / plain_s, "rounds": plain_rounds}
+        print(f"[bench] plain: {plain_s:.2f} s for the batch ({plain_rounds} loop rounds over 4 runs)",
+              file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(engine, per_class, cfg, args.cpu_tasks)
+        print(f"[bench] cpu: {cpu['value']:.3f} samples/s on {cpu['cores']} threads ({cpu['seconds']:.1f} s)",
+              file=sys.stderr, flush=True)
+
+    achieved = attempts * BYTES_PER_ATTEMPT / kern_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": samples_per_s,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "6x6 periodic RBIM p=0.04, 100 disorder samples x 4 error classes (400 tasks), "
+                        "10 intervals overlap 0.25, 8 walkers/interval, alpha 0.8, ln f 1 -> 1e-6, "
+                        "exchange every 20 rounds, 1000 attempts/walker/round; batched persistent kernel",
+            "tasks": int(batch.tasks.size),
+            "tasks_finished": n_ok,
+        },
+        "attempts_per_s": attempts / kern_s,
+        "attempts_per_step": attempts,
+        "kernel_ms": float(np.mean(kernel_ms)),
+        "plain_samples_per_s": plain["samples_per_s"] if plain else None,
+        "batched_vs_plain": (samples_per_s / plain["samples_per_s"]) if plain else None,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -165,6 +165,12 @@ int wl_wang_landau(int8_t* d_lattice, const int8_t* d_interactions, uint64_t* d_
 int wl_fisher_yates(int* d_shuffle, uint64_t* d_seed_offset, int seed, int total_intervals, int walker_per_interval,
                     int walker_per_interactions, void* stream);
 
+/* The device's Metropolis decision u < min(1, exp(d)), u = (x + 1/2) 2^-32,
+ * for n pairs (d_delta[i], d_words[i]) -> d_out[i] in {0, 1}: the sweep
+ * kernels' FP32-estimate-with-exact-fallback path, exposed so that it can be
+ * checked against the exact rule.  No counterpart in the reference. */
+int wl_accept_decisions(const double* d_delta, const uint32_t* d_words, int n, int8_t* d_out, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Part 2: drivers                                                      */
 /* ------------------------------------------------------------------ */
@@ -216,8 +222,35 @@ size_t wl_fused_task_lds_bytes(const wl_task* task, const int* h_start, const in
  * reachable energies in LDS; workgroups pull tasks from a device queue.
  * d_tasks is a device copy of the task table.  Per task: d_counter is the
  * exchange counter (start at 1), d_rounds accumulates rounds, d_status is
- * 0 = running, 1 = finished, 2 = new energy, 3 = too large for one workgroup.
+ * 0 = running, 1 = finished, 2 = new energy, 3 = too large for one workgroup,
+ * 4 = a histogram count did not fit 32 bits, 5 = invalid walker state (an
+ * energy off the task's reachable grid, a replica index or lattice slot out
+ * of range).
  * Bit-identical to wl_mainrun_loop on the same task with the same rounds. */
+/* Launch shape of the batched kernel for a task table (host side). */
+typedef struct wl_fused_launch_plan {
+  int num_tasks;
+  int grid;             /* persistent workgroups (<= resident capacity) */
+  int block;            /* threads: largest task's walkers rounded to 64 */
+  int lds_bytes;        /* dynamic LDS: the largest task's plan */
+  int resident_per_cu;  /* workgroups of this shape resident per CU */
+  int num_cu;
+} wl_fused_launch_plan;
+
+/* Checks every task fits one workgroup and fills the launch plan;
+ * h_start/h_end are the host interval tables the tasks index into. */
+int wl_fused_prepare(const wl_task* h_tasks, int num_tasks, const int* h_start, const int* h_end,
+                     wl_fused_launch_plan* plan);
+
+/* Enqueues the batched kernel on `stream` without synchronising: the
+ * caller owns d_queue (one int of device memory, reset by this call). */
+int wl_fused_launch(const wl_fused_launch_plan* plan, const wl_task* d_tasks, int8_t* d_lattice,
+                    const int8_t* d_interactions, uint64_t* d_H, double* d_logG, double* d_factor, int* d_energy,
+                    uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx, int8_t* d_cond, int* d_cond_sample,
+                    const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
+                    int64_t* d_rounds, int* d_status, int* d_queue, int64_t max_rounds, void* stream);
+
+/* Convenience: prepare + launch + synchronise (allocates the queue counter). */
 int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, int8_t* d_lattice,
                  const int8_t* d_interactions, uint64_t* d_H, double* d_logG, double* d_factor, int* d_energy,
                  uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx, int8_t* d_cond, int* d_cond_sample,

@@ -7,12 +7,17 @@
 // the checker for the HIP engine: only tests, the smoke check and the CPU
 // baseline of the benchmark may load it.  Nothing in the product links it.
 //
-// Random numbers, exp/log/sqrt: the engine defines its stream as
+// Random numbers, exp/log/sqrt: the engine defines its streams as
 // Philox4x32-10 (Salmon et al., SC'11) keyed (seed + sample, tag) with
-// counter (draw/4, slot) and word draw%4 mapped to (x + 1/2) 2^-32; exp is
-// Cody-Waite reduction + degree-13 Taylor, log is the atanh series, sqrt is a
-// Newton reciprocal-square-root iteration.  They are restated here from those
-// published methods so that the oracle and the device produce the same bits.
+// counter (block, slot).  WL attempt a of a walker uses words 2(a&1) (site)
+// and 2(a&1)+1 (acceptance) of block a/2 of its walk stream; its shuffle and
+// exchange draws use word d%4 of block d/4 of its exchange stream; its
+// offset_iter packs attempts (low 40 bits) and exchange draws (high 24).
+// A word x gives the index floor(x n / 2^32) or the uniform (x + 1/2) 2^-32.
+// exp is Cody-Waite reduction + degree-13 Taylor, log is the atanh series,
+// sqrt is a Newton reciprocal-square-root iteration.  They are restated here
+// from those published methods so that the oracle and the device produce the
+// same bits.
 //
 // Deviations from the reference that the engine defines (each is also a
 // deviation of the engine, so both sides agree):
@@ -29,7 +34,11 @@
 //  D10 exchange parity is the interval index within the sample unless
 //      compat mode asks for the global block index (src/wl.cu:830);
 //  D12 ln f is log(f) of the stored factor, computed with the shared log;
-//  D13 exchange acceptance is min(1, exp(a + b)) (reference exp(a)*exp(b), src/wl.cu:872).
+//  D13 exchange acceptance is min(1, exp(a + b)) (reference exp(a)*exp(b), src/wl.cu:872);
+//  D14 every WL attempt consumes two words (the reference draws the second
+//      only for in-range proposals, src/wl.cu:1114), shuffle/exchange draws
+//      come from a separate stream, and indices are floor(x n / 2^32)
+//      (reference trunc(u (n - 1 + 0.999999)), src/wl.cu:66).
 //
 // Parity status per function is listed in the header of tests/test_oracle.py.
 
@@ -80,22 +89,55 @@ void philox10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
   std::memcpy(out, c, sizeof c);
 }
 
-double uniform_at(int key0, uint32_t tag, uint32_t slot, uint64_t draw)
+void block_at(int key0, uint32_t tag, uint32_t slot, uint64_t blk, uint32_t out[4])
 {
-  const uint64_t blk = draw / 4;
   const uint32_t ctr[4] = {static_cast<uint32_t>(blk & 0xffffffffu), static_cast<uint32_t>(blk >> 32), slot, 0u};
   const uint32_t key[2] = {static_cast<uint32_t>(key0), tag};
-  uint32_t out[4];
   philox10(ctr, key, out);
-  return (static_cast<double>(out[draw % 4]) + 0.5) / 4294967296.0;
 }
 
-constexpr uint32_t TAG_WALK = 0, TAG_DISORDER = 1, TAG_LATTICE = 2, TAG_PRERUN = 3;
-
-// trunc(u * (n - 1 + 0.999999)), src/wl.cu:66-68 and :800-802.
-int pick(double u, int n)
+uint32_t word_at(int key0, uint32_t tag, uint32_t slot, uint64_t draw)
 {
-  return static_cast<int>(u * (static_cast<double>(n - 1) + 0.999999));
+  uint32_t out[4];
+  block_at(key0, tag, slot, draw / 4, out);
+  return out[draw % 4];
+}
+
+double unit(uint32_t x)
+{
+  return (static_cast<double>(x) + 0.5) / 4294967296.0;
+}
+
+double uniform_at(int key0, uint32_t tag, uint32_t slot, uint64_t draw)
+{
+  return unit(word_at(key0, tag, slot, draw));
+}
+
+constexpr uint32_t TAG_WALK = 0, TAG_DISORDER = 1, TAG_LATTICE = 2, TAG_PRERUN = 3, TAG_EXCHANGE = 4;
+constexpr uint64_t ATTEMPT_MASK = (1ull << 40) - 1;
+constexpr uint64_t EXCHANGE_ONE = 1ull << 40;
+
+// floor(x n / 2^32): the index a word selects among n.
+int pick(uint32_t x, int n)
+{
+  return static_cast<int>((static_cast<uint64_t>(x) * static_cast<uint64_t>(n)) >> 32);
+}
+
+// Site and acceptance words of WL attempt a of a walker.
+void attempt_words(int key0, uint32_t slot, uint64_t a, uint32_t& xs, uint32_t& xa)
+{
+  uint32_t out[4];
+  block_at(key0, TAG_WALK, slot, a / 2, out);
+  xs = out[2 * (a % 2)];
+  xa = out[2 * (a % 2) + 1];
+}
+
+// Exchange draw of a walker: advances the high part of its counter.
+uint32_t exchange_draw(int key0, uint32_t slot, uint64_t& counter)
+{
+  const uint32_t x = word_at(key0, TAG_EXCHANGE, slot, counter >> 40);
+  counter += EXCHANGE_ONE;
+  return x;
 }
 
 double from_bits(uint64_t b)
@@ -257,9 +299,13 @@ WLO_API double wlo_uniform(int key0, uint32_t tag, uint32_t slot, uint64_t draw)
 {
   return uniform_at(key0, tag, slot, draw);
 }
-WLO_API int wlo_pick(double u, int n)
+WLO_API int wlo_pick(uint32_t x, int n)
 {
-  return pick(u, n);
+  return pick(x, n);
+}
+WLO_API uint32_t wlo_word(int key0, uint32_t tag, uint32_t slot, uint64_t draw)
+{
+  return word_at(key0, tag, slot, draw);
 }
 WLO_API double wlo_exp(double x)
 {
@@ -391,9 +437,9 @@ WLO_API void wlo_wang_landau_pre_run(int* E,
       for (int w = 0; w < Wp; ++w) {
         const int64_t t = static_cast<int64_t>(s) * Wp + w;
         const int8_t* lat = lattice + off_lattice[t];
-        const double u = uniform_at(seed + s, TAG_PRERUN, static_cast<uint32_t>(w), seed_offset[t]);
+        const uint32_t xs = word_at(seed + s, TAG_PRERUN, static_cast<uint32_t>(w), seed_offset[t]);
         seed_offset[t] += 1;
-        const int sidx = pick(u, N);
+        const int sidx = pick(xs, N);
         site[w] = sidx;
         newE[w] = flip_energy(lat, Js, X, Y, boundary, E[t], sidx / Y, sidx % Y);
         const double prob = exp_det(static_cast<double>(H[E[t] - E_min]) - static_cast<double>(H[newE[w] - E_min]));
@@ -618,14 +664,12 @@ WLO_API void wlo_replica_exchange(int* off_lattice,
     if (cond_sample[s] == -1) {
       continue;
     }
-    // Fisher-Yates by walker 0 of the interval.
+    // Fisher-Yates by walker 0 of the interval (src/wl.cu:785-808).
     {
       const int64_t t0 = static_cast<int64_t>(k) * W;
       const uint32_t slot = static_cast<uint32_t>(t0 % walker_per_interactions);
       for (int i = W - 1; i > 0; --i) {
-        const double u = uniform_at(seed + s, TAG_WALK, slot, off_iter[t0]);
-        off_iter[t0] += 1;
-        const int r = pick(u, i + 1);
+        const int r = pick(exchange_draw(seed + s, slot, off_iter[t0]), i + 1);
         std::swap(idx[t0 + i], idx[t0 + r]);
       }
     }
@@ -644,12 +688,10 @@ WLO_API void wlo_replica_exchange(int* off_lattice,
       const double gjEX = logG[off_hist[sw] + E[t] - start[k + 1]];
       const double p = accept_p((giEX - giEY) + (gjEY - gjEX));
       const uint32_t slot = static_cast<uint32_t>(t % walker_per_interactions);
-      const double u = uniform_at(seed + s, TAG_WALK, slot, off_iter[t]);
-      if (u < p) {
+      if (unit(exchange_draw(seed + s, slot, off_iter[t])) < p) {
         std::swap(off_lattice[t], off_lattice[sw]);
         std::swap(E[t], E[sw]);
       }
-      off_iter[t] += 1;
     }
   }
 }
@@ -753,9 +795,10 @@ WLO_API void wlo_wang_landau(int8_t* lattice,
     const int emax = end[s * num_intervals + num_intervals - 1];
     const double lnf = log_det(factor[t]);
     for (int it = 0; it < num_iterations; ++it) {
-      const double u = uniform_at(seed + s, TAG_WALK, slot, off_iter[t]);
+      uint32_t xs, xa;
+      attempt_words(seed + s, slot, off_iter[t] & ATTEMPT_MASK, xs, xa);
       off_iter[t] += 1;
-      const int sidx = pick(u, N);
+      const int sidx = pick(xs, N);
       const int i = sidx / Y, j = sidx % Y;
       const int en = flip_energy(lat, Js, X, Y, boundary, E[t], i, j);
       if (en < emin || en > emax || spectrum[off_spec[s] + en - emin] != 1) {
@@ -772,9 +815,7 @@ WLO_API void wlo_wang_landau(int8_t* lattice,
         } else {
           const int in = off_hist[t] + en - start[k];
           const double p = accept_p(logG[io] - logG[in]);
-          const double u2 = uniform_at(seed + s, TAG_WALK, slot, off_iter[t]);
-          off_iter[t] += 1;
-          if (u2 < p) {
+          if (unit(xa) < p) {
             lat[sidx] *= -1;
             H[in] += 1;
             logG[in] += lnf;
@@ -789,9 +830,7 @@ WLO_API void wlo_wang_landau(int8_t* lattice,
           const int io = off_hist[t] + E[t] - start[k];
           const int in = off_hist[t] + en - start[k];
           const double p = accept_p(logG[io] - logG[in]);
-          const double u2 = uniform_at(seed + s, TAG_WALK, slot, off_iter[t]);
-          off_iter[t] += 1;
-          if (u2 < p) {
+          if (unit(xa) < p) {
             lat[sidx] *= -1;
             E[t] = en;
           }

@@ -22,12 +22,14 @@ OUT = PKG / "_lib"
 OBJ = PKG / "_build"
 INCLUDE = ROOT / "include"
 
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", f"{ROCM}/bin/hipcc")
 ARCH = os.environ.get("WL_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["wl_plain.hip", "wl_fused.hip"]
 HOST_SOURCES = ["wl_host.cpp"]
-EXECUTABLES: dict[str, str] = {}
+EXECUTABLES = {"prerun": "prerun_main.cpp", "mainrun": "mainrun_main.cpp"}
+CLI_SOURCES = ["wl_io.cpp"]
 
 COMMON = [
     "-O3",
@@ -93,7 +95,7 @@ def _compile(src: Path, obj: Path, device: bool, extra: list[str]) -> Path:
     if device:
         cmd[1:1] = DEVICE_FLAGS
     else:
-        cmd[1:1] = ["-x", "c++"]
+        cmd[1:1] = ["-x", "c++", f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
     _run(cmd)
     return obj
 
@@ -113,8 +115,10 @@ def build(verbose: bool = False) -> Path:
         exe_objs = {}
         for exe, src in EXECUTABLES.items():
             exe_objs[exe] = ex.submit(_compile, CSRC / src, OBJ / (src + ".o"), False, h5_c)
+        cli_objs = [ex.submit(_compile, CSRC / src, OBJ / (src + ".o"), False, h5_c) for src in CLI_SOURCES]
         objs = [j.result() for j in jobs]
         exe_objs = {k: v.result() for k, v in exe_objs.items()}
+        cli_objs = [j.result() for j in cli_objs]
     lib = OUT / "libwl_amd.so"
     tmp = OUT / "libwl_amd.so.tmp"
     _run([HIPCC, *DEVICE_FLAGS, "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
@@ -122,8 +126,8 @@ def build(verbose: bool = False) -> Path:
     bindir = OUT / "bin"
     bindir.mkdir(exist_ok=True)
     for exe, obj in exe_objs.items():
-        _run([HIPCC, *DEVICE_FLAGS, "-o", str(bindir / exe), str(obj), f"-L{OUT}", "-lwl_amd",
-              "-Wl,-rpath,$ORIGIN/..", *h5_l])
+        _run([HIPCC, *DEVICE_FLAGS, "-o", str(bindir / exe), str(obj), *map(str, cli_objs), f"-L{OUT}",
+              "-lwl_amd", "-Wl,-rpath,$ORIGIN/..", *h5_l])
     if verbose:
         print(f"built {lib} (hdf5: {'yes' if h5 else 'no'})")
     return lib

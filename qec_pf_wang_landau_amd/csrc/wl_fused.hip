@@ -170,7 +170,8 @@ __device__ __forceinline__ int div_small(int n, uint32_t recip)
   return static_cast<int>(__umulhi(static_cast<uint32_t>(n), recip));
 }
 
-__device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned char* lds, int64_t max_rounds)
+__device__ __forceinline__ void run_task(const wl_task& t, const task_state& st, int q, unsigned char* lds,
+                                         int64_t max_rounds)
 {
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
@@ -218,7 +219,7 @@ __device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned
   int* EL = lds_at<int>(lds, p.o_E);
   int* SL = lds_at<int>(lds, p.o_slot);
   int* IL = lds_at<int>(lds, p.o_idx);
-  int* misc = lds_at<int>(lds, p.o_misc);  // [0] new-energy flag, [1] finished, [2] overflow
+  int* misc = lds_at<int>(lds, p.o_misc);  // [0] new energy, [1] finished, [2] H overflow, [3] bad input
 
   const int8_t* Jg = st.J + t.couplings_off;
   const int64_t wbase = t.walker_off;
@@ -255,6 +256,7 @@ __device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned
     misc[0] = 0;
     misc[1] = 0;
     misc[2] = 0;
+    misc[3] = 0;
   }
   for (int l = tid; l < M; l += nthr) {
     const int8_t* src = st.lattice + t.lattice_off + static_cast<int64_t>(l) * N;
@@ -286,16 +288,38 @@ __device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned
       }
     }
   }
+  // Inputs that would index outside the task's LDS are rejected: every
+  // walker's energy must be a spectrum energy inside its interval, replica
+  // indices must lie in [0, W) and lattice slots in [0, M).
+  for (int l = tid; l < M; l += nthr) {
+    const int kk = l / W;
+    const int e = EL[l];
+    const bool ok_e = e >= Lsk[kk] && e <= Lek[kk] && ((e - emin) & (stride - 1)) == 0 && mask[e - emin] == 1;
+    const bool ok_i = IL[l] >= 0 && IL[l] < W;
+    const bool ok_s = SL[l] >= 0 && SL[l] < M;
+    if (!(ok_e && ok_i && ok_s)) {
+      misc[3] = 1;
+    }
+  }
   __syncthreads();
+  if (misc[3] != 0) {
+    if (tid == 0) {
+      st.status[q] = 5;
+    }
+    __syncthreads();
+    return;
+  }
 
   // ---- per-lane walker registers ---------------------------------------
   const bool walker = tid < M;
   const int k = walker ? tid / W : 0;
   const int w = walker ? tid - k * W : 0;
   const uint32_t recipY = static_cast<uint32_t>((0x100000000ull + Y - 1) / Y);
-  const int per_sample = M;
-  stream_reader rng;
-  rng.init(t.rng_key, kTagWalk, static_cast<uint32_t>(tid % per_sample), walker ? st.off_iter[wbase + tid] : 0);
+  const uint64_t c0 = walker ? st.off_iter[wbase + tid] : 0;
+  uint64_t att = c0 & kAttemptMask;        // WL attempts so far
+  uint64_t xdraws = c0 >> kAttemptBits;    // exchange-stream draws so far
+  attempt_stream rng;
+  rng.init(t.rng_key, static_cast<uint32_t>(tid), att);
   const double ebeta = wl_exp(t.beta);
   const bool compat = (t.flags & 1) != 0;
   int64_t counter = st.counter[q];
@@ -316,7 +340,10 @@ __device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned
       int b_cur = (e_cur - e0) >> shift;
       double g_cur = Gw[b_cur * W];
       for (int it = 0; it < t.num_iterations; ++it) {
-        const int site = pick_index(rng.next(), N);
+        uint32_t xs, xa;
+        rng.words(att, xs, xa);
+        ++att;
+        const int site = pick_index(xs, static_cast<uint32_t>(N));
         const int i = div_small(site, recipY);
         const int j = site - i * Y;
         const int ipp = (i + 1 < X) ? i + 1 : 0;
@@ -345,31 +372,23 @@ __device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned
           break;
         }
         const bool in_range = en >= sk && en <= ek;
-        if (!converged) {
-          if (in_range) {
-            const int b_new = (en - e0) >> shift;
-            const double g_new = Gw[b_new * W];
-            const double pacc = accept_prob(g_cur - g_new);
-            if (rng.next() < pacc) {
-              L[i] = rc ^ (1u << j);
-              e_cur = en;
-              b_cur = b_new;
-              g_cur = g_new;
-            }
-          }
-          g_cur = g_cur + lnf;
-          Gw[b_cur * W] = g_cur;
-          Hw[b_cur * W] += 1u;
-        } else if (in_range) {
+        // The walker owns its histogram row and its lattice slot, so the
+        // increment and the flip are LDS atomics without return (no
+        // read-modify-write round trip on the dependent chain).
+        if (in_range) {
           const int b_new = (en - e0) >> shift;
           const double g_new = Gw[b_new * W];
-          const double pacc = accept_prob(g_cur - g_new);
-          if (rng.next() < pacc) {
-            L[i] = rc ^ (1u << j);
+          if (accept_fast(g_cur - g_new, xa)) {
+            atomicXor(&L[i], 1u << j);
             e_cur = en;
             b_cur = b_new;
             g_cur = g_new;
           }
+        }
+        if (!converged) {
+          g_cur = g_cur + lnf;
+          Gw[b_cur * W] = g_cur;
+          atomicAdd(&Hw[b_cur * W], 1u);
         }
       }
       EL[tid] = e_cur;
@@ -474,12 +493,15 @@ __device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned
         const bool part = walker && k < I - 1 && ((even && par == 0) || (!even && par != 0));
         if (part && w == 0) {
           int* seg = IL + k * W;
+          stream_reader xr;
+          xr.init(t.rng_key, kTagExchange, static_cast<uint32_t>(tid), xdraws);
           for (int ii = W - 1; ii > 0; --ii) {
-            const int r = pick_index(rng.next(), ii + 1);
+            const int r = pick_index(xr.next_u32(), static_cast<uint32_t>(ii + 1));
             const int tmp = seg[ii];
             seg[ii] = seg[r];
             seg[r] = tmp;
           }
+          xdraws = xr.draw;
         }
         __syncthreads();
         if (part) {
@@ -494,8 +516,9 @@ __device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned
             const double giEY = Gi[((Es - Le0[k]) >> shift) * W];
             const double gjEY = Gj[((Es - Le0[k + 1]) >> shift) * W];
             const double gjEX = Gj[((Et - Le0[k + 1]) >> shift) * W];
-            const double pacc = accept_prob((giEX - giEY) + (gjEY - gjEX));
-            if (rng.next() < pacc) {
+            const uint32_t x = exchange_word(t.rng_key, static_cast<uint32_t>(tid), xdraws);
+            ++xdraws;
+            if (accept_exact((giEX - giEY) + (gjEY - gjEX), x)) {
               const int sl = SL[tid];
               SL[tid] = SL[pt];
               SL[pt] = sl;
@@ -529,7 +552,7 @@ __device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned
     st.factor[wbase + l] = Lf[kk];
   }
   if (walker) {
-    st.off_iter[wbase + tid] = rng.draw;
+    st.off_iter[wbase + tid] = att | (xdraws << kAttemptBits);
   }
   for (int l = tid; l < M; l += nthr) {
     const int kk = l / W, ww = l - (l / W) * W;
@@ -555,8 +578,12 @@ __device__ void run_task(const wl_task& t, const task_state& st, int q, unsigned
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(kMaxBlock) k_fused(const wl_task* tasks, int num_tasks, int* queue, task_state st,
-                                                     int64_t max_rounds)
+// MAXB is the largest block the variant is compiled for: the 256-thread
+// variant keeps the whole task body in registers (up to 512 VGPRs per lane);
+// the 1024-thread variant serves tasks with more than 256 walkers.
+template<int MAXB>
+__global__ void __launch_bounds__(MAXB) k_fused(const wl_task* tasks, int num_tasks, int* queue, task_state st,
+                                                int64_t max_rounds)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_task;
@@ -594,6 +621,13 @@ __global__ void k_zero_int(int* p)
 
 using namespace wlamd;
 
+namespace {
+std::string g_fused_err()
+{
+  return std::string(wl_last_error());
+}
+}  // namespace
+
 extern "C" {
 
 size_t wl_fused_task_lds_bytes(const wl_task* task, const int* h_start, const int* h_end)
@@ -603,6 +637,69 @@ size_t wl_fused_task_lds_bytes(const wl_task* task, const int* h_start, const in
   }
   lds_plan p{};
   return host_plan(*task, h_start, h_end, p) ? static_cast<size_t>(p.bytes) : 0;
+}
+
+int wl_fused_prepare(const wl_task* h_tasks, int num_tasks, const int* h_start, const int* h_end,
+                     wl_fused_launch_plan* plan)
+{
+  return guarded([&] {
+    require(num_tasks >= 0 && h_tasks != nullptr && plan != nullptr, "fused_prepare: bad task table");
+    int max_block = 64;
+    size_t max_lds = 0;
+    for (int q = 0; q < num_tasks; ++q) {
+      const wl_task& t = h_tasks[q];
+      require(t.exchange_every > 0 && t.num_iterations >= 0, "fused_prepare: bad task parameters");
+      require(t.energy_stride == 1 || t.energy_stride == 2 || t.energy_stride == 4,
+              "fused_prepare: energy_stride must be 1, 2 or 4");
+      lds_plan p{};
+      if (!host_plan(t, h_start + t.interval_off, h_end + t.interval_off, p)) {
+        throw std::invalid_argument("fused_prepare: task " + std::to_string(q) + " does not fit one workgroup");
+      }
+      max_block = std::max(max_block, round_up64(p.M));
+      max_lds = std::max(max_lds, static_cast<size_t>(p.bytes));
+    }
+    int dev = 0;
+    WL_HIP(hipGetDevice(&dev));
+    int num_cu = 0;
+    WL_HIP(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    int per_cu = 0;
+    const void* kern = max_block <= 256 ? reinterpret_cast<const void*>(k_fused<256>)
+                                        : reinterpret_cast<const void*>(k_fused<kMaxBlock>);
+    WL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, max_block, max_lds));
+    per_cu = std::max(per_cu, 1);
+    plan->num_tasks = num_tasks;
+    plan->block = max_block;
+    plan->lds_bytes = static_cast<int>(max_lds);
+    plan->grid = std::max(1, std::min(num_tasks, num_cu * per_cu));
+    plan->resident_per_cu = per_cu;
+    plan->num_cu = num_cu;
+  });
+}
+
+int wl_fused_launch(const wl_fused_launch_plan* plan, const wl_task* d_tasks, int8_t* d_lattice,
+                    const int8_t* d_interactions, uint64_t* d_H, double* d_logG, double* d_factor, int* d_energy,
+                    uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx, int8_t* d_cond, int* d_cond_sample,
+                    const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
+                    int64_t* d_rounds, int* d_status, int* d_queue, int64_t max_rounds, void* stream)
+{
+  return guarded([&] {
+    require(plan != nullptr && d_queue != nullptr, "fused_launch: missing plan or queue counter");
+    if (plan->num_tasks == 0) {
+      return;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    task_state s{d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy, d_offset_iter, d_offset_lattice,
+                 d_idx, d_cond, d_cond_sample, d_start, d_end, d_spectrum, d_counter, d_rounds, d_status};
+    k_zero_int<<<1, 1, 0, st>>>(d_queue);
+    if (plan->block <= 256) {
+      k_fused<256><<<plan->grid, plan->block, plan->lds_bytes, st>>>(d_tasks, plan->num_tasks, d_queue, s,
+                                                                     max_rounds);
+    } else {
+      k_fused<kMaxBlock><<<plan->grid, plan->block, plan->lds_bytes, st>>>(d_tasks, plan->num_tasks, d_queue, s,
+                                                                           max_rounds);
+    }
+    WL_LAUNCH_CHECK();
+  });
 }
 
 int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, int8_t* d_lattice,
@@ -617,49 +714,28 @@ int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, 
       return;
     }
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    // Size the launch from the largest task: block = walkers rounded to a
-    // wave, LDS = the largest plan.
-    int max_block = 64;
-    size_t max_lds = 0;
-    int max_intervals_total = 0;
+    int n_iv = 0;
     for (int q = 0; q < num_tasks; ++q) {
-      max_intervals_total = std::max(max_intervals_total, h_tasks[q].interval_off + h_tasks[q].I);
+      n_iv = std::max(n_iv, h_tasks[q].interval_off + h_tasks[q].I);
     }
-    std::vector<int> h_start(max_intervals_total), h_end(max_intervals_total);
-    WL_HIP(hipMemcpyAsync(h_start.data(), d_start, sizeof(int) * max_intervals_total, hipMemcpyDeviceToHost, st));
-    WL_HIP(hipMemcpyAsync(h_end.data(), d_end, sizeof(int) * max_intervals_total, hipMemcpyDeviceToHost, st));
+    std::vector<int> h_start(n_iv), h_end(n_iv);
+    WL_HIP(hipMemcpyAsync(h_start.data(), d_start, sizeof(int) * n_iv, hipMemcpyDeviceToHost, st));
+    WL_HIP(hipMemcpyAsync(h_end.data(), d_end, sizeof(int) * n_iv, hipMemcpyDeviceToHost, st));
     WL_HIP(hipStreamSynchronize(st));
-    for (int q = 0; q < num_tasks; ++q) {
-      const wl_task& t = h_tasks[q];
-      require(t.exchange_every > 0 && t.num_iterations >= 0, "fused_run: bad task parameters");
-      require(t.energy_stride == 1 || t.energy_stride == 2 || t.energy_stride == 4,
-              "fused_run: energy_stride must be 1, 2 or 4");
-      lds_plan p{};
-      if (!host_plan(t, h_start.data() + t.interval_off, h_end.data() + t.interval_off, p)) {
-        throw std::invalid_argument("fused_run: task " + std::to_string(q) + " does not fit one workgroup");
-      }
-      max_block = std::max(max_block, round_up64(p.M));
-      max_lds = std::max(max_lds, static_cast<size_t>(p.bytes));
+    wl_fused_launch_plan plan{};
+    if (wl_fused_prepare(h_tasks, num_tasks, h_start.data(), h_end.data(), &plan) != 0) {
+      throw std::invalid_argument(g_fused_err());
     }
-    int dev = 0;
-    WL_HIP(hipGetDevice(&dev));
-    int num_cu = 0;
-    WL_HIP(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    int per_cu = 0;
-    WL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_fused), max_block,
-                                                         max_lds));
-    per_cu = std::max(per_cu, 1);
-    const int grid = std::min(num_tasks, num_cu * per_cu);
     int* d_queue = nullptr;
     WL_HIP(hipMalloc(&d_queue, sizeof(int)));
-    task_state s{d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy, d_offset_iter, d_offset_lattice,
-                 d_idx, d_cond, d_cond_sample, d_start, d_end, d_spectrum, d_counter, d_rounds, d_status};
-    k_zero_int<<<1, 1, 0, st>>>(d_queue);
-    k_fused<<<grid, max_block, max_lds, st>>>(d_tasks, num_tasks, d_queue, s, max_rounds);
-    const hipError_t launch = hipGetLastError();
+    const int rc = wl_fused_launch(&plan, d_tasks, d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy,
+                                   d_offset_iter, d_offset_lattice, d_idx, d_cond, d_cond_sample, d_start, d_end,
+                                   d_spectrum, d_counter, d_rounds, d_status, d_queue, max_rounds, stream);
     const hipError_t sync = hipStreamSynchronize(st);
     (void)hipFree(d_queue);
-    hip_check(launch, __FILE__, __LINE__);
+    if (rc != 0) {
+      throw std::runtime_error(g_fused_err());
+    }
     hip_check(sync, __FILE__, __LINE__);
   });
 }

@@ -10,11 +10,18 @@
 // Random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
 // numbers: as easy as 1, 2, 3", SC'11).  The reference draws from cuRAND's
 // Philox4_32_10 streams keyed by (seed + sample, subsequence = walker slot,
-// offset = draw counter) (src/wl.cu:1074); here the same three coordinates
-// key a counter-based Philox block directly:
+// offset = draw counter) (src/wl.cu:1074); here the same coordinates key a
+// counter-based Philox block directly:
 //   key     = (seed + sample, stream tag)
-//   counter = (draw / 4 as 64 bits, walker slot, 0)
-//   uniform = word (draw % 4) of that block, mapped to (0, 1).
+//   counter = (block index as 64 bits, walker slot, 0)
+// A Wang-Landau attempt a of a walker reads words 2(a&1) and 2(a&1)+1 of
+// block a/2 of its walk stream (site word, acceptance word), whether or not
+// the acceptance word is used, so walkers that started together stay on the
+// same block and one Philox evaluation serves two attempts of a whole wave.
+// The shuffle and exchange draws of a walker come from its exchange stream,
+// word d%4 of block d/4 for its d-th exchange draw.  A walker's two counters
+// share one u64 (the reference's offset_iter): attempts in the low 40 bits,
+// exchange draws above.
 //
 // exp: Cody & Waite argument reduction (Cody, Waite, "Software Manual for the
 // Elementary Functions", 1980) with the fdlibm two-part ln 2, followed by a
@@ -39,11 +46,16 @@ namespace wlamd {
 
 // Stream tags (second Philox key word).
 enum : uint32_t {
-  kTagWalk = 0u,      // mainrun walker slot streams (WL sweep, shuffle, exchange)
+  kTagWalk = 0u,      // mainrun walker slot streams: WL attempts
   kTagDisorder = 1u,  // coupling signs, one draw per bond
   kTagLattice = 2u,   // prerun initial lattices
   kTagPrerun = 3u,    // prerun walker streams
+  kTagExchange = 4u,  // mainrun walker slot streams: shuffle and exchange
 };
+
+// Packed per-walker counter: attempts (low 40 bits), exchange draws (high 24).
+constexpr uint64_t kAttemptBits = 40;
+constexpr uint64_t kAttemptMask = (1ull << kAttemptBits) - 1ull;
 
 WL_HD uint64_t dbl_bits(double x)
 {
@@ -107,6 +119,15 @@ WL_HD philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, 
   return philox4{{c0, c1, c2, c3}};
 }
 
+// Word `sel` (0..3) of a block, without a dynamic register index (which
+// would put the block in scratch memory on the device).
+WL_HD uint32_t select_word(const philox4& b, uint32_t sel)
+{
+  const uint32_t lo = (sel & 1u) ? b.v[1] : b.v[0];
+  const uint32_t hi = (sel & 1u) ? b.v[3] : b.v[2];
+  return (sel & 2u) ? hi : lo;
+}
+
 // Map a 32-bit word to a double in (0, 1): (x + 1/2) * 2^-32, exact.
 WL_HD double u32_to_unit(uint32_t x)
 {
@@ -123,7 +144,7 @@ WL_HD philox4 stream_block(int seed_plus_sample, uint32_t tag, uint32_t slot, ui
 WL_HD double stream_uniform(int seed_plus_sample, uint32_t tag, uint32_t slot, uint64_t draw)
 {
   const philox4 b = stream_block(seed_plus_sample, tag, slot, draw >> 2);
-  return u32_to_unit(b.v[draw & 3u]);
+  return u32_to_unit(select_word(b, static_cast<uint32_t>(draw & 3u)));
 }
 
 // Sequential reader of one stream that keeps the current Philox block in
@@ -143,25 +164,81 @@ struct stream_reader {
     block = ~0ull;
   }
 
-  WL_HD double next()
+  WL_HD uint32_t next_u32()
   {
     const uint64_t b = draw >> 2;
     if (b != block) {
       cache = philox4x32_10(static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32), slot, 0u, key0, tag);
       block = b;
     }
-    const uint32_t w = cache.v[draw & 3u];
+    const uint32_t w = select_word(cache, static_cast<uint32_t>(draw & 3u));
     ++draw;
-    return u32_to_unit(w);
+    return w;
+  }
+
+  WL_HD double next() { return u32_to_unit(next_u32()); }
+};
+
+// Uniform index in [0, n) from a 32-bit word: floor(x n / 2^32).  (The
+// reference forms trunc(u * (n - 1 + 0.999999)) from a float uniform,
+// src/wl.cu:66-68; the integer form is exactly uniform and one instruction.)
+WL_HD int pick_index(uint32_t x, uint32_t n)
+{
+  return static_cast<int>((static_cast<uint64_t>(x) * n) >> 32);
+}
+
+// Words of WL attempt `a` of walker `slot`: one Philox block per two attempts,
+// cached, and the next block evaluated one pair ahead.
+struct attempt_stream {
+  uint32_t key0, slot;
+  uint64_t blk;      // index of `cur`
+  philox4 cur, nxt;  // blocks blk and blk + 1
+
+  WL_HD philox4 eval(uint64_t b) const
+  {
+    return philox4x32_10(static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32), slot, 0u, key0, kTagWalk);
+  }
+
+  WL_HD void init(int seed_plus_sample, uint32_t slot_, uint64_t attempt)
+  {
+    key0 = static_cast<uint32_t>(seed_plus_sample);
+    slot = slot_;
+    blk = attempt >> 1;
+    cur = eval(blk);
+    nxt = eval(blk + 1);
+  }
+
+  // Site and acceptance words of attempt a (a >= the attempt given to init,
+  // non-decreasing between calls).
+  WL_HD void words(uint64_t a, uint32_t& xs, uint32_t& xa)
+  {
+    const uint64_t b = a >> 1;
+    if (b != blk) {
+      if (b == blk + 1) {
+        cur = nxt;
+      } else {
+        cur = eval(b);
+      }
+      blk = b;
+      nxt = eval(b + 1);
+    }
+    const bool second = (a & 1u) != 0;
+    xs = second ? cur.v[2] : cur.v[0];
+    xa = second ? cur.v[3] : cur.v[1];
   }
 };
 
-// Random site in [0, n) exactly as the reference forms it from a uniform:
-// trunc(u * (n - 1 + 0.999999)) (src/wl.cu:66-68).
-WL_HD int pick_index(double u, int n)
+WL_HD void attempt_words(int seed_plus_sample, uint32_t slot, uint64_t a, uint32_t& xs, uint32_t& xa)
 {
-  const double scale = static_cast<double>(n - 1) + 0.999999;
-  return static_cast<int>(u * scale);
+  const philox4 b = stream_block(seed_plus_sample, kTagWalk, slot, a >> 1);
+  xs = (a & 1u) ? b.v[2] : b.v[0];
+  xa = (a & 1u) ? b.v[3] : b.v[1];
+}
+
+// Exchange-stream word d of a walker (shuffle and exchange draws).
+WL_HD uint32_t exchange_word(int seed_plus_sample, uint32_t slot, uint64_t d)
+{
+  return select_word(stream_block(seed_plus_sample, kTagExchange, slot, d >> 2), static_cast<uint32_t>(d & 3u));
 }
 
 // 2^k for -1022 <= k <= 1023 from the exponent bits.
@@ -267,5 +344,43 @@ WL_HD double accept_prob(double d)
 {
   return d >= 0.0 ? 1.0 : wl_exp(d);
 }
+
+// The acceptance decision u < min(1, exp(d)) with u = (x + 1/2) 2^-32.
+WL_HD bool accept_exact(double d, uint32_t x)
+{
+  return u32_to_unit(x) < accept_prob(d);
+}
+
+#if defined(__HIPCC__) || defined(__HIP__)
+// Same decision as accept_exact, faster on the device: exp(d) is first
+// estimated in FP32 with the hardware exp2; the FP64 comparison is only made
+// when u lies within a relative 2^-12 band around the estimate.  The estimate
+// is within 2^-16 relative of exp(d) for -80 <= d < 0 (FP32 rounding of
+// d log2 e contributes < 2^-17 for |d log2 e| < 128, v_exp_f32 1 ulp), so
+// outside the band both comparisons agree; below -80, exp(d) < 2^-115 is
+// smaller than every u (>= 2^-33), and both reject.
+__device__ __forceinline__ bool accept_fast(double d, uint32_t x)
+{
+  if (d >= 0.0) {
+    return true;  // u < 1 always
+  }
+  if (d < -80.0) {
+    return false;
+  }
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float e = __builtin_amdgcn_exp2f(static_cast<float>(d * 0x1.71547652b82fep+0));
+#else
+  const float e = 0.0f;  // the host pass never executes device code
+#endif
+  const float u = static_cast<float>(u32_to_unit(x));
+  if (u < e * (1.0f - 0x1p-12f)) {
+    return true;
+  }
+  if (u > e * (1.0f + 0x1p-12f)) {
+    return false;
+  }
+  return u32_to_unit(x) < wl_exp(d);
+}
+#endif
 
 }  // namespace wlamd

@@ -175,7 +175,7 @@ __global__ void k_prerun(int* E, int* flag_found, int8_t* lattice, int8_t* found
     bool acc = false;
     int site = 0, en = 0;
     if (active) {
-      site = pick_index(rng.next(), N);
+      site = pick_index(rng.next_u32(), static_cast<uint32_t>(N));
       en = flip_energy_i8(lat, Js, X, Y, boundary, e_cur, site / Y, site % Y);
       const double prob =
         wl_exp(static_cast<double>(H[e_cur - E_min]) - static_cast<double>(H[en - E_min]));
@@ -334,18 +334,21 @@ __global__ void k_redistribute(double* logG, const double* shared, const int* st
   }
 }
 
+// Shuffle of one interval's replica indices by its walker 0, from that
+// walker's exchange stream (src/wl.cu:785-808).
 __device__ void fisher_yates_segment(int* idx, uint64_t* off_iter, int seed_plus_sample, int64_t t0, uint32_t slot,
                                      int W)
 {
+  const uint64_t c = off_iter[t0];
   stream_reader rng;
-  rng.init(seed_plus_sample, kTagWalk, slot, off_iter[t0]);
+  rng.init(seed_plus_sample, kTagExchange, slot, c >> kAttemptBits);
   for (int i = W - 1; i > 0; --i) {
-    const int r = pick_index(rng.next(), i + 1);
+    const int r = pick_index(rng.next_u32(), static_cast<uint32_t>(i + 1));
     const int tmp = idx[t0 + i];
     idx[t0 + i] = idx[t0 + r];
     idx[t0 + r] = tmp;
   }
-  off_iter[t0] = rng.draw;
+  off_iter[t0] = (c & kAttemptMask) | (rng.draw << kAttemptBits);
 }
 
 __global__ void k_fisher_yates(int* idx, uint64_t* off_iter, int seed, int total_intervals, int W, int per_sample)
@@ -401,16 +404,16 @@ __global__ void k_replica_exchange(int* off_lattice, int* E, int* idx, uint64_t*
   const double giEY = logG[off_hist[t] + Es - start[k]];
   const double gjEY = logG[off_hist[sw] + Es - start[k + 1]];
   const double gjEX = logG[off_hist[sw] + Et - start[k + 1]];
-  const double p = accept_prob((giEX - giEY) + (gjEY - gjEX));
-  const double u = stream_uniform(seed + s, kTagWalk, static_cast<uint32_t>(t % per_sample), off_iter[t]);
-  if (u < p) {
+  const uint64_t c = off_iter[t];
+  const uint32_t x = exchange_word(seed + s, static_cast<uint32_t>(t % per_sample), c >> kAttemptBits);
+  if (accept_exact((giEX - giEY) + (gjEY - gjEX), x)) {
     const int ol = off_lattice[t];
     off_lattice[t] = off_lattice[sw];
     off_lattice[sw] = ol;
     E[t] = Es;
     E[sw] = Et;
   }
-  off_iter[t] += 1;
+  off_iter[t] = c + (1ull << kAttemptBits);
 }
 
 // Block per interval, one lane per walker: min/sum over the spectrum bins,
@@ -503,12 +506,17 @@ __global__ void __launch_bounds__(kBlock) k_wang_landau(int8_t* lattice, const i
   double* g = logG + off_hist[t] - sk;
   const double lnf = wl_log(factor[t]);
   const bool converged = cond[k] != 0;
-  stream_reader rng;
-  rng.init(seed + s, kTagWalk, static_cast<uint32_t>(t % per_sample), off_iter[t]);
+  const uint64_t c0 = off_iter[t];
+  uint64_t a = c0 & kAttemptMask;
+  attempt_stream rng;
+  rng.init(seed + s, static_cast<uint32_t>(t % per_sample), a);
   int e_cur = Earr[t];
   int flag = 0;
   for (int it = 0; it < num_iterations; ++it) {
-    const int site = pick_index(rng.next(), N);
+    uint32_t xs, xa;
+    rng.words(a, xs, xa);
+    ++a;
+    const int site = pick_index(xs, static_cast<uint32_t>(N));
     const int i = site / Y;
     const int j = site - i * Y;
     const int en = flip_energy_i8(lat, Js, X, Y, boundary, e_cur, i, j);
@@ -520,26 +528,20 @@ __global__ void __launch_bounds__(kBlock) k_wang_landau(int8_t* lattice, const i
     const bool in_range = en >= sk && en <= ek;
     if (!converged) {
       int upd = e_cur;
-      if (in_range) {
-        const double p = accept_prob(g[e_cur] - g[en]);
-        if (rng.next() < p) {
-          lat[site] = static_cast<int8_t>(-lat[site]);
-          e_cur = en;
-          upd = en;
-        }
+      if (in_range && accept_fast(g[e_cur] - g[en], xa)) {
+        lat[site] = static_cast<int8_t>(-lat[site]);
+        e_cur = en;
+        upd = en;
       }
       h[upd] += 1;
       g[upd] += lnf;
-    } else if (in_range) {
-      const double p = accept_prob(g[e_cur] - g[en]);
-      if (rng.next() < p) {
-        lat[site] = static_cast<int8_t>(-lat[site]);
-        e_cur = en;
-      }
+    } else if (in_range && accept_fast(g[e_cur] - g[en], xa)) {
+      lat[site] = static_cast<int8_t>(-lat[site]);
+      e_cur = en;
     }
   }
   Earr[t] = e_cur;
-  off_iter[t] = rng.draw;
+  off_iter[t] = (c0 & ~kAttemptMask) | a;
   if (num_iterations > 0) {
     found[t] = flag;
   }
@@ -577,6 +579,14 @@ __global__ void k_round_status(const int* found, const double* factor, int total
   if (threadIdx.x == 0) {
     status[0] = static_cast<unsigned long long>(any);
     status[1] = mx;
+  }
+}
+
+__global__ void k_accept(const double* d, const uint32_t* x, int n, int8_t* out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    out[i] = accept_fast(d[i], x[i]) ? 1 : 0;
   }
 }
 
@@ -868,6 +878,17 @@ int wl_wang_landau(int8_t* d_lattice, const int8_t* d_J, uint64_t* d_H, double* 
       d_lattice, d_J, d_H, d_logG, factor, d_energy, d_newE, d_off_iter, foundFlag, d_start, d_end, d_off_hist,
       d_off_lattice, d_cond, d_spectrum, d_off_spec, d_cond_sample, X, Y, num_iterations, num_lattices, I, per_sample,
       seed, boundary);
+    WL_LAUNCH_CHECK();
+  });
+}
+
+int wl_accept_decisions(const double* d_delta, const uint32_t* d_words, int n, int8_t* d_out, void* stream)
+{
+  return guarded([&] {
+    if (n <= 0) {
+      return;
+    }
+    k_accept<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(d_delta, d_words, n, d_out);
     WL_LAUNCH_CHECK();
   });
 }

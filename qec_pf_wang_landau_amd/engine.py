@@ -445,3 +445,124 @@ def run_fused(ds: DeviceState, tasks: np.ndarray, max_rounds: int = 1 << 40, d_t
     call("wl_fused_run", dptr(d_tasks), _p(tasks), tasks.size, ds["lattice"], ds["J"], ds["H"], ds["logG"],
          ds["factor"], ds["E"], ds["off_iter"], ds["off_lattice"], ds["idx"], ds["cond"], ds["cond_sample"],
          ds["start"], ds["end"], ds["spectrum"], ds["counter"], ds["rounds"], ds["status"], max_rounds, None)
+
+
+# ------------------------------------------------------- GPU problem setup --
+
+class Keep:
+    """Holds device tensors for the duration of queued kernels."""
+
+    def __init__(self):
+        self.items = []
+
+    def __call__(self, a):
+        t = to_device(np.ascontiguousarray(a))
+        self.items.append(t)
+        return t
+
+
+def disorder_gpu(seed: int, S: int, X: int, Y: int, prob: float, cls: str) -> np.ndarray:
+    """src/prerun.cu:52-75 on the GPU: couplings of S samples with the class chain."""
+    torch = _torch()
+    N = X * Y
+    J = torch.zeros(S * 2 * N, dtype=torch.int8, device="cuda")
+    call("wl_init_interactions", dptr(J), N, S, seed, C.c_float(prob), None)
+    if cls in ("h", "c"):
+        call("wl_apply_x_horizontal_error", dptr(J), X, Y, S, None)
+    if cls in ("v", "c"):
+        call("wl_apply_x_vertical_error", dptr(J), X, Y, S, None)
+    return to_host(J, np.int8)
+
+
+def prerun_gpu(J: np.ndarray, X: int, Y: int, boundary: int, seed: int, S: int, walkers: int, iters: int,
+               num_intervals: int):
+    """src/prerun.cu:25-133 on the GPU: per sample (binary histogram over
+    [-2N, 2N], found lattices, their energies)."""
+    torch = _torch()
+    N = X * Y
+    keep = Keep()
+    iv = interval_results(-2 * N, 2 * N, num_intervals, 1, 1.0)
+    total = S * walkers
+    lat = torch.zeros(total * N, dtype=torch.int8, device="cuda")
+    probs = torch.zeros(total, dtype=torch.float32, device="cuda")
+    off = torch.zeros(total, dtype=torch.int32, device="cuda")
+    E = torch.zeros(total, dtype=torch.int32, device="cuda")
+    tJ = keep(J)
+    call("wl_init_lattice", dptr(lat), dptr(probs), X, Y, total, seed, walkers, None)
+    call("wl_init_offsets_lattice", dptr(off), X, Y, total, None)
+    call("wl_calc_energy", boundary, dptr(E), dptr(lat), dptr(tJ), dptr(off), X, Y, total, walkers, None)
+    hist = torch.zeros(S * (4 * N + 1), dtype=torch.int64, device="cuda")
+    so = torch.zeros(total, dtype=torch.int64, device="cuda")
+    flag = torch.zeros(S * num_intervals, dtype=torch.int32, device="cuda")
+    found = torch.zeros(S * num_intervals * N, dtype=torch.int8, device="cuda")
+    call("wl_wang_landau_pre_run", dptr(E), dptr(flag), dptr(lat), dptr(found), dptr(hist), dptr(so), dptr(tJ),
+         dptr(off), -2 * N, 2 * N, iters, X, Y, seed, num_intervals, iv.len_interval, total, walkers, boundary, None)
+    h_hist = to_host(hist, np.uint64)
+    h_found = to_host(found, np.int8)
+    out = []
+    for s in range(S):
+        h = (h_hist[s * (4 * N + 1):(s + 1) * (4 * N + 1)] != 0).astype(np.uint8)
+        fl = h_found[s * num_intervals * N:(s + 1) * num_intervals * N].reshape(num_intervals, N)
+        fl = fl[fl[:, 0] != 0]
+        fe = (lattice_energies(fl, J[s * 2 * N:(s + 1) * 2 * N], X, Y, boundary) if fl.size
+              else np.zeros(0, np.int32))
+        out.append((h, fl.copy(), np.asarray(fe, np.int32)))
+    return out
+
+
+def problem_gpu(cfg: RunConfig, S: int, prob: float, cls: str, prerun_walkers: int = 128,
+                prerun_iters: int = 10000, prerun_intervals: int = 16, sample_base: int = 0
+                ) -> list[SampleInput]:
+    """Disorder + prerun on the GPU -> one SampleInput per sample; sample q
+    uses seed + q for its disorder and its walker streams (the reference keys
+    everything by seed + sample, src/wl.cu:380,1074)."""
+    X, Y = cfg.X, cfg.Y
+    N = X * Y
+    seed = cfg.seed + sample_base
+    J = disorder_gpu(seed, S, X, Y, prob, cls)
+    pre = prerun_gpu(J, X, Y, cfg.boundary, seed, S, prerun_walkers, prerun_iters, prerun_intervals)
+    out = []
+    for s in range(S):
+        h, fl, fe = pre[s]
+        emin, emax, mask = energy_spectrum(h, X, Y)
+        out.append(SampleInput(J=J[s * 2 * N:(s + 1) * 2 * N].copy(), e_min=emin, e_max=emax, mask=mask, found=fl,
+                               found_energy=fe, rng_key=seed + s))
+    return out
+
+
+class FusedLauncher:
+    """Prepared batched launch over a BatchDevice: restore + enqueue without
+    host synchronisation, for timing and graph capture."""
+
+    def __init__(self, dev: BatchDevice):
+        torch = _torch()
+        self.dev = dev
+        b = dev.batch
+        self.plan = np.zeros(6, np.int32)
+        call("wl_fused_prepare", _p(b.tasks), b.tasks.size, _p(b.arrays["start"]), _p(b.arrays["end"]),
+             _p(self.plan))
+        self.queue = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.pristine = {k: v.clone() for k, v in dev.t.items()}
+
+    @property
+    def grid(self) -> int:
+        return int(self.plan[1])
+
+    @property
+    def block(self) -> int:
+        return int(self.plan[2])
+
+    @property
+    def lds_bytes(self) -> int:
+        return int(self.plan[3])
+
+    def restore(self) -> None:
+        for k, v in self.pristine.items():
+            self.dev.t[k].copy_(v)
+
+    def launch(self, max_rounds: int = 1 << 40, stream=None) -> None:
+        d = self.dev
+        call("wl_fused_launch", _p(self.plan), dptr(d.d_tasks), d["lattice"], d["J"], d["H"], d["logG"],
+             d["factor"], d["E"], d["off_iter"], d["off_lattice"], d["idx"], d["cond"], d["cond_sample"], d["start"],
+             d["end"], d["spectrum"], d["counter"], d["rounds"], d["status"], dptr(self.queue), max_rounds,
+             stream)

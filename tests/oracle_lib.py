@@ -25,7 +25,8 @@ F64 = C.c_double
 SIG = {
     "wlo_philox4x32_10": (None, [P, P, P]),
     "wlo_uniform": (F64, [I32, U32, U32, U64]),
-    "wlo_pick": (I32, [F64, I32]),
+    "wlo_pick": (I32, [U32, I32]),
+    "wlo_word": (U32, [I32, U32, U32, U64]),
     "wlo_exp": (F64, [F64]),
     "wlo_log": (F64, [F64]),
     "wlo_sqrt": (F64, [F64]),

@@ -19,8 +19,22 @@ from qec_pf_wang_landau_amd.engine import dptr, to_device, to_host
 pytestmark = pytest.mark.gpu
 
 
+_KEEP: list = []
+
+
+@pytest.fixture(autouse=True)
+def _keep_alive():
+    """Device tensors made inline for a call stay referenced until the test
+    ends, so the caching allocator cannot hand their memory to the next
+    argument while a kernel is still queued on it."""
+    yield
+    _KEEP.clear()
+
+
 def d(a):
-    return to_device(np.ascontiguousarray(a))
+    t = to_device(np.ascontiguousarray(a))
+    _KEEP.append(t)
+    return t
 
 
 def h(t, like):
@@ -51,6 +65,31 @@ def test_math_host_device_bit_exact(cuda):
         expect = np.array([oracle().wlo_sqrt(float(x)) for x in f])
         assert np.array_equal(got, expect)
         f = got
+
+
+def test_fast_acceptance_equals_exact_rule(cuda):
+    """The sweep's FP32 estimate + exact fallback decides exactly as
+    u < min(1, exp(d)) computed in FP64 by the oracle, including words
+    placed within a few ulps of the boundary."""
+    rng = np.random.default_rng(11)
+    d = np.concatenate([rng.uniform(-90, 5, 200000), -rng.exponential(1.0, 200000), [0.0, -0.0, -80.0, -80.0001]])
+    x = rng.integers(0, 2**32, d.size, dtype=np.uint64).astype(np.uint32)
+    # words adjacent to the boundary: x such that (x + 1/2) 2^-32 ~ exp(d)
+    for i in range(0, 100000):
+        e = math.exp(d[i]) if d[i] < 0 else 1.0
+        xb = int(e * 2**32 - 0.5)
+        x[i] = np.uint32(min(max(xb + (i % 5) - 2, 0), 2**32 - 1))
+    o = oracle()
+    expect = np.array([1 if (float(xi) + 0.5) / 2**32 < (1.0 if di >= 0 else o.wlo_exp(float(di))) else 0
+                       for di, xi in zip(d, x)], np.int8)
+    out = d_(np.zeros(d.size, np.int8))
+    call("wl_accept_decisions", dptr(d_(d)), dptr(d_(x.view(np.int32))), d.size, dptr(out), None)
+    got = h(out, expect)
+    assert np.array_equal(got, expect), np.nonzero(got != expect)[0][:10]
+
+
+def d_(a):
+    return d(a)
 
 
 @pytest.mark.parametrize("boundary", [0, 1])

@@ -66,11 +66,14 @@ def test_uniform_is_open_unit_interval_and_uses_all_words(oracle):
     assert [oracle.wlo_uniform(7, 0, 3, d) for d in range(4)] == [(float(x) + 0.5) / 2**32 for x in out]
 
 
-def test_pick_matches_reference_formula(oracle):
-    for u in [1e-9, 0.3, 0.5, 0.999999999]:
-        for n in [2, 16, 36, 1024]:
-            assert oracle.wlo_pick(u, n) == int(u * ((n - 1) + 0.999999))
-    assert oracle.wlo_pick(1.0 - 2**-33, 16) == 15
+def test_pick_is_floor_of_scaled_word(oracle):
+    """Index from a word: floor(x n / 2^32) -- every index gets floor or ceil
+    of 2^32 / n words, the first and last words map to 0 and n - 1."""
+    for n in [1, 2, 3, 16, 36, 37, 1024]:
+        assert oracle.wlo_pick(0, n) == 0 and oracle.wlo_pick(2**32 - 1, n) == n - 1
+        for k in range(1, n):
+            first = -(-k * 2**32 // n)  # smallest word with index k
+            assert oracle.wlo_pick(first, n) == k and oracle.wlo_pick(first - 1, n) == k - 1
 
 
 def _ulp_err(a, b):
@@ -377,7 +380,7 @@ def test_single_wl_step_golden(boundary):
         expG[t * L + d["E"][t] + 2 * N] = 1.0
     assert np.array_equal(d["H"], expH)
     assert np.max(np.abs(d["logG"] - expG)) < 1e-15
-    assert (d["off_iter"] == 2).all()
+    assert (d["off_iter"] == 1).all()  # one attempt
     assert (d["E"] != E0).any()
 
 
@@ -406,15 +409,18 @@ def exchange_setup():
                 off_iter=np.zeros(S * I * W, np.uint64))
 
 
+TAG_EXCHANGE = 4
+
+
 def expected_permutations(seed, S, I, W, oracle_lib_oracle):
-    """The shuffle thread 0 of each interval performs, from counter 0."""
+    """The shuffle walker 0 of each interval performs from its exchange stream, from counter 0."""
     perms = []
     for k in range(S * I):
         idx = list(range(W))
         t0 = k * W
         for d, i in enumerate(range(W - 1, 0, -1)):
-            u = oracle_lib_oracle.wlo_uniform(seed + t0 // (I * W), 0, t0 % (I * W), d)
-            r = oracle_lib_oracle.wlo_pick(u, i + 1)
+            x = oracle_lib_oracle.wlo_word(seed + t0 // (I * W), TAG_EXCHANGE, t0 % (I * W), d)
+            r = oracle_lib_oracle.wlo_pick(x, i + 1)
             idx[i], idx[r] = idx[r], idx[i]
         perms.append(idx)
     return perms
@@ -440,7 +446,7 @@ def test_replica_exchange_flat_log_g_swaps_every_pair(oracle, even):
                 assert d["off_lat"][t] == L0[sw] and d["off_lat"][sw] == L0[t]
                 assert d["E"][t] == E0[sw] and d["E"][sw] == E0[t]
                 expect_draws = (W - 1 if j == 0 else 0) + 1
-                assert d["off_iter"][t] == expect_draws
+                assert d["off_iter"][t] == expect_draws << 40  # exchange draws live in the high bits
             elif not (iid > 0 and (iid - 1) % 2 == (0 if even else 1)):
                 assert d["off_lat"][t] == L0[t]
 
