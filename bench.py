@@ -141,6 +141,7 @@ def main():
     status = result.arrays["status"]
     n_ok = int(np.sum(status == 1))
     attempts = attempts_of(result.arrays, batch.tasks)
+    rounds = result.arrays["rounds"].astype(np.int64)
     step_s = wall / args.steps
     kern_s = float(np.mean(kernel_ms)) / 1e3
     samples_per_s = args.samples / step_s
@@ -190,6 +191,8 @@ def main():
             "tasks_finished": n_ok,
         },
         "attempts_per_s": attempts / kern_s,
+        "task_rounds": {"max": int(rounds.max()), "mean": float(rounds.mean()), "p50": float(np.median(rounds))},
+        "tail_ns_per_attempt": kern_s / (int(rounds.max()) * cfg.num_iterations) * 1e9,
         "attempts_per_step": attempts,
         "kernel_ms": float(np.mean(kernel_ms)),
         "plain_samples_per_s": plain["samples_per_s"] if plain else None,

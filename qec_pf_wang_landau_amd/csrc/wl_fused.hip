@@ -163,11 +163,82 @@ __device__ __forceinline__ T* lds_at(unsigned char* base, int off)
   return reinterpret_cast<T*>(base + off);
 }
 
-// Unsigned division by a small constant through a 32-bit reciprocal:
-// exact for n * d < 2^26 (sites of a lattice with at most 1024 spins).
-__device__ __forceinline__ int div_small(int n, uint32_t recip)
+
+// Walker registers carried across attempts.
+struct sweep_state {
+  int e;       // energy
+  int b;       // compact bin of e
+  double g;    // ln g at b (mirrors the LDS value)
+  bool alive;  // no energy outside the spectrum seen yet
+};
+
+// Loop-invariant inputs of one walker's sweep.
+struct sweep_ctx {
+  uint32_t* L;            // lattice rows (bits) of the walker's slot
+  const uint32_t* Jh;     // right-bond signs per row
+  const uint32_t* Jv;     // down-bond signs per row
+  const int8_t* mask;     // spectrum flags over [E_min, E_max]
+  double* G;              // walker's ln g column (stride W)
+  uint32_t* H;            // walker's histogram column (stride W)
+  int X, Y, N, boundary;
+  int emin, erange;       // E_min, E_max - E_min
+  int sk, ek, e0, shift, W, nbm1;
+  bool upd;               // interval still active
+  double lnf;
+  float invY;
+};
+
+// One Wang-Landau attempt (src/wl.cu:1076-1174) on LDS state, branch-free
+// except for the rare exact acceptance fallback: both LDS reads that depend
+// on the proposed energy (spectrum flag, ln g at the new bin) are issued
+// together with clamped indices, and every effect is a select or an LDS
+// atomic whose operand is zero when the effect does not apply.
+__device__ __forceinline__ void attempt(const sweep_ctx& c, sweep_state& s, uint32_t xs, uint32_t xa)
 {
-  return static_cast<int>(__umulhi(static_cast<uint32_t>(n), recip));
+  const int site = pick_index(xs, static_cast<uint32_t>(c.N));
+  // site / Y in FP32: (site + 1/2) / Y is at least 1/(2Y) away from an integer.
+  const int i = static_cast<int>((static_cast<float>(site) + 0.5f) * c.invY);
+  const int j = site - static_cast<int>(__umul24(static_cast<uint32_t>(i), static_cast<uint32_t>(c.Y)));
+  const int ipp = (i + 1 < c.X) ? i + 1 : 0;
+  const int inn = (i > 0) ? i - 1 : c.X - 1;
+  const int jpp = (j + 1 < c.Y) ? j + 1 : 0;
+  const int jnn = (j > 0) ? j - 1 : c.Y - 1;
+  const uint32_t rc = c.L[i], ru = c.L[inn], rd = c.L[ipp];
+  const uint32_t jh = c.Jh[i], jvu = c.Jv[inn], jvd = c.Jv[i];
+  const uint32_t sbit = (rc >> j) & 1u;
+  // mismatch (s_nb J_nb = -1) per neighbour
+  const uint32_t mu = ((ru ^ jvu) >> j) & 1u;
+  const uint32_t md = ((rd ^ jvd) >> j) & 1u;
+  const uint32_t ml = ((rc >> jnn) ^ (jh >> jnn)) & 1u;
+  const uint32_t mr = ((rc >> jpp) ^ (jh >> j)) & 1u;
+  int sum;
+  if (c.boundary == 1) {
+    const int cu = i > 0, cd = i < c.X - 1, cl = j > 0, cr = j < c.Y - 1;
+    sum = cu + cd + cl + cr - 2 * static_cast<int>(cu * mu + cd * md + cl * ml + cr * mr);
+  } else {
+    sum = 4 - 2 * static_cast<int>(mu + md + ml + mr);
+  }
+  const int neg = -static_cast<int>(sbit);              // 0 or -1
+  const int en = s.e + (((2 * sum) ^ neg) - neg);        // E + 2 s sum, s = +1/-1
+  const int rel = en - c.emin;
+  const bool in_spec_range = static_cast<uint32_t>(rel) <= static_cast<uint32_t>(c.erange);
+  const bool in_range = (en >= c.sk) & (en <= c.ek);
+  // clamped so that the speculative read stays inside the walker's column
+  const int b_new = in_range ? min(max((en - c.e0) >> c.shift, 0), c.nbm1) : s.b;
+  const int8_t flag = c.mask[in_spec_range ? rel : 0];
+  const double g_new = c.G[__mul24(b_new, c.W)];
+  const bool valid = s.alive & in_spec_range & (flag == 1);
+  const bool acc = valid & in_range & accept_fast(s.g - g_new, xa);
+  atomicXor(&c.L[i], acc ? (1u << j) : 0u);
+  s.e = acc ? en : s.e;
+  s.b = acc ? b_new : s.b;
+  s.g = acc ? g_new : s.g;
+  const bool upd = valid & c.upd;
+  s.g = upd ? s.g + c.lnf : s.g;
+  const int at = __mul24(s.b, c.W);
+  c.G[at] = s.g;  // unchanged value when !upd
+  atomicAdd(&c.H[at], upd ? 1u : 0u);
+  s.alive = valid;
 }
 
 __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st, int q, unsigned char* lds,
@@ -314,18 +385,20 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
   const bool walker = tid < M;
   const int k = walker ? tid / W : 0;
   const int w = walker ? tid - k * W : 0;
-  const uint32_t recipY = static_cast<uint32_t>((0x100000000ull + Y - 1) / Y);
+  const float invY = 1.0f / static_cast<float>(Y);
   const uint64_t c0 = walker ? st.off_iter[wbase + tid] : 0;
   uint64_t att = c0 & kAttemptMask;        // WL attempts so far
   uint64_t xdraws = c0 >> kAttemptBits;    // exchange-stream draws so far
-  attempt_stream rng;
-  rng.init(t.rng_key, static_cast<uint32_t>(tid), att);
+  attempt_stream rng;  // only its key and eval() are used: blocks are fetched per pair below
+  rng.key0 = static_cast<uint32_t>(t.rng_key);
+  rng.slot = static_cast<uint32_t>(tid);
   const double ebeta = wl_exp(t.beta);
   const bool compat = (t.flags & 1) != 0;
   int64_t counter = st.counter[q];
   int64_t rounds_done = 0;
   int status = 0;
   const int8_t* maskE = mask - emin;  // maskE[E]
+  const int8_t* maskE0 = mask;        // maskE0[E - emin]
 
   while (rounds_done < max_rounds) {
     // ---- sweep: num_iterations WL attempts per walker -----------------
@@ -333,65 +406,43 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       const int sk = Lsk[k], ek = Lek[k], e0 = Le0[k], gb = Lgb[k];
       double* Gw = G + gb + w;
       uint32_t* Hw = Hc + gb + w;
-      const bool converged = Lcond[k] != 0;
+      const bool upd = Lcond[k] == 0;  // active interval: ln g and H are updated
       const double lnf = wl_log(Lf[k]);
       uint32_t* L = lat + SL[tid] * X;
-      int e_cur = EL[tid];
-      int b_cur = (e_cur - e0) >> shift;
-      double g_cur = Gw[b_cur * W];
-      for (int it = 0; it < t.num_iterations; ++it) {
-        uint32_t xs, xa;
-        rng.words(att, xs, xa);
+      sweep_state ws{EL[tid], 0, 0.0, true};
+      ws.b = (ws.e - e0) >> shift;
+      ws.g = Gw[ws.b * W];
+      const sweep_ctx cx{L,  Jh, Jv, maskE0, Gw, Hw,    X,      Y,          N,   t.boundary, emin, emax - emin, sk,
+                         ek, e0, shift, W, Lnb[k] - 1, upd, lnf, invY};
+      int it = 0;
+      const int n = t.num_iterations;
+      uint64_t blk = att >> 1;
+      philox4 cur = rng.eval(blk);
+      if ((att & 1u) && it < n) {  // odd start: second half of the block
+        attempt(cx, ws, cur.v[2], cur.v[3]);
         ++att;
-        const int site = pick_index(xs, static_cast<uint32_t>(N));
-        const int i = div_small(site, recipY);
-        const int j = site - i * Y;
-        const int ipp = (i + 1 < X) ? i + 1 : 0;
-        const int inn = (i - 1 >= 0) ? i - 1 : X - 1;
-        const int jpp = (j + 1 < Y) ? j + 1 : 0;
-        const int jnn = (j - 1 >= 0) ? j - 1 : Y - 1;
-        const uint32_t rc = L[i], ru = L[inn], rd = L[ipp];
-        const uint32_t jh = Jh[i], jvu = Jv[inn], jvd = Jv[i];
-        const uint32_t sbit = (rc >> j) & 1u;
-        // mismatch (s_nb * J == -1) per neighbour
-        const uint32_t mu = ((ru >> j) ^ (jvu >> j)) & 1u;
-        const uint32_t md = ((rd >> j) ^ (jvd >> j)) & 1u;
-        const uint32_t ml = ((rc >> jnn) ^ (jh >> jnn)) & 1u;
-        const uint32_t mr = ((rc >> jpp) ^ (jh >> j)) & 1u;
-        int sum;
-        if (t.boundary == 1) {
-          const int cu = i > 0, cd = i < X - 1, cl = j > 0, cr = j < Y - 1;
-          sum = cu * (1 - 2 * static_cast<int>(mu)) + cd * (1 - 2 * static_cast<int>(md)) +
-                cl * (1 - 2 * static_cast<int>(ml)) + cr * (1 - 2 * static_cast<int>(mr));
-        } else {
-          sum = 4 - 2 * static_cast<int>(mu + md + ml + mr);
-        }
-        const int en = e_cur + (sbit ? -2 * sum : 2 * sum);
-        if (en < emin || en > emax || maskE[en] != 1) {
-          misc[0] = 1;
-          break;
-        }
-        const bool in_range = en >= sk && en <= ek;
-        // The walker owns its histogram row and its lattice slot, so the
-        // increment and the flip are LDS atomics without return (no
-        // read-modify-write round trip on the dependent chain).
-        if (in_range) {
-          const int b_new = (en - e0) >> shift;
-          const double g_new = Gw[b_new * W];
-          if (accept_fast(g_cur - g_new, xa)) {
-            atomicXor(&L[i], 1u << j);
-            e_cur = en;
-            b_cur = b_new;
-            g_cur = g_new;
-          }
-        }
-        if (!converged) {
-          g_cur = g_cur + lnf;
-          Gw[b_cur * W] = g_cur;
-          atomicAdd(&Hw[b_cur * W], 1u);
-        }
+        ++it;
+        ++blk;
+        cur = rng.eval(blk);
       }
-      EL[tid] = e_cur;
+      // Two attempts per Philox block; the next block is evaluated while
+      // the current pair waits on LDS.
+      for (; it + 2 <= n && ws.alive; it += 2) {
+        const philox4 nx = rng.eval(blk + 1);
+        attempt(cx, ws, cur.v[0], cur.v[1]);
+        attempt(cx, ws, cur.v[2], cur.v[3]);
+        att += 2;
+        ++blk;
+        cur = nx;
+      }
+      if (it < n && ws.alive) {
+        attempt(cx, ws, cur.v[0], cur.v[1]);
+        ++att;
+      }
+      if (!ws.alive) {
+        misc[0] = 1;
+      }
+      EL[tid] = ws.e;
     }
     for (int kk = tid; kk < I; kk += nthr) {
       Lvote[kk] = 0;

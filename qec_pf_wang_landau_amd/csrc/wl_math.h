@@ -353,33 +353,31 @@ WL_HD bool accept_exact(double d, uint32_t x)
 
 #if defined(__HIPCC__) || defined(__HIP__)
 // Same decision as accept_exact, faster on the device: exp(d) is first
-// estimated in FP32 with the hardware exp2; the FP64 comparison is only made
-// when u lies within a relative 2^-12 band around the estimate.  The estimate
-// is within 2^-16 relative of exp(d) for -80 <= d < 0 (FP32 rounding of
-// d log2 e contributes < 2^-17 for |d log2 e| < 128, v_exp_f32 1 ulp), so
-// outside the band both comparisons agree; below -80, exp(d) < 2^-115 is
-// smaller than every u (>= 2^-33), and both reject.
+// estimated in FP32 with the hardware exp2 and u in FP32; the FP64 comparison
+// is only made when u lies within a relative 2^-12 band around the estimate.
+// The estimate is within 2^-16 relative of exp(d) for -80 <= d < 0 (FP32
+// rounding of d log2 e contributes < 2^-17 for |d log2 e| < 128, v_exp_f32
+// 1 ulp) and the FP32 u within 2^-23 of u, so outside the band both
+// comparisons agree; below -80, exp(d) < 2^-115 is smaller than every u
+// (>= 2^-33), and both reject.  Only the band case branches.
 __device__ __forceinline__ bool accept_fast(double d, uint32_t x)
 {
-  if (d >= 0.0) {
-    return true;  // u < 1 always
-  }
-  if (d < -80.0) {
-    return false;
-  }
 #if defined(__HIP_DEVICE_COMPILE__)
   const float e = __builtin_amdgcn_exp2f(static_cast<float>(d * 0x1.71547652b82fep+0));
 #else
   const float e = 0.0f;  // the host pass never executes device code
 #endif
-  const float u = static_cast<float>(u32_to_unit(x));
-  if (u < e * (1.0f - 0x1p-12f)) {
-    return true;
+  const float u = __builtin_fmaf(static_cast<float>(x), 0x1p-32f, 0x1p-33f);
+  const bool pos = d >= 0.0;
+  const bool tiny = d < -80.0;
+  const bool fa = u < e * (1.0f - 0x1p-12f);
+  const bool fr = u > e * (1.0f + 0x1p-12f);
+  bool acc = pos | (!tiny & fa);
+  const bool amb = !pos & !tiny & !fa & !fr;
+  if (amb) {
+    acc = u32_to_unit(x) < wl_exp(d);
   }
-  if (u > e * (1.0f + 0x1p-12f)) {
-    return false;
-  }
-  return u32_to_unit(x) < wl_exp(d);
+  return acc;
 }
 #endif
 
