@@ -142,6 +142,8 @@ def main():
     n_ok = int(np.sum(status == 1))
     attempts = attempts_of(result.arrays, batch.tasks)
     rounds = result.arrays["rounds"].astype(np.int64)
+    task_s = launcher.task_seconds()  # last timed step (restore() clears the ticks)
+    slowest = int(np.argmax(rounds))
     step_s = wall / args.steps
     kern_s = float(np.mean(kernel_ms)) / 1e3
     samples_per_s = args.samples / step_s
@@ -193,6 +195,8 @@ def main():
         "attempts_per_s": attempts / kern_s,
         "task_rounds": {"max": int(rounds.max()), "mean": float(rounds.mean()), "p50": float(np.median(rounds))},
         "tail_ns_per_attempt": kern_s / (int(rounds.max()) * cfg.num_iterations) * 1e9,
+        "slowest_task_ns_per_attempt": float(task_s[slowest] / (rounds[slowest] * cfg.num_iterations) * 1e9),
+        "task_seconds": {"max": float(task_s.max()), "mean": float(task_s.mean())},
         "attempts_per_step": attempts,
         "kernel_ms": float(np.mean(kernel_ms)),
         "plain_samples_per_s": plain["samples_per_s"] if plain else None,

@@ -235,6 +235,7 @@ typedef struct wl_fused_launch_plan {
   int lds_bytes;        /* dynamic LDS: the largest task's plan */
   int resident_per_cu;  /* workgroups of this shape resident per CU */
   int num_cu;
+  int clock_khz;        /* rate of the per-task tick counter */
 } wl_fused_launch_plan;
 
 /* Checks every task fits one workgroup and fills the launch plan;
@@ -243,12 +244,15 @@ int wl_fused_prepare(const wl_task* h_tasks, int num_tasks, const int* h_start, 
                      wl_fused_launch_plan* plan);
 
 /* Enqueues the batched kernel on `stream` without synchronising: the
- * caller owns d_queue (one int of device memory, reset by this call). */
+ * caller owns d_queue (one int of device memory, reset by this call).
+ * d_task_ticks (optional, may be NULL) accumulates per task the device
+ * clock ticks (plan->clock_khz per ms) its workgroup spent on it. */
 int wl_fused_launch(const wl_fused_launch_plan* plan, const wl_task* d_tasks, int8_t* d_lattice,
                     const int8_t* d_interactions, uint64_t* d_H, double* d_logG, double* d_factor, int* d_energy,
                     uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx, int8_t* d_cond, int* d_cond_sample,
                     const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
-                    int64_t* d_rounds, int* d_status, int* d_queue, int64_t max_rounds, void* stream);
+                    int64_t* d_rounds, int* d_status, int* d_queue, int64_t* d_task_ticks, int64_t max_rounds,
+                    void* stream);
 
 /* Convenience: prepare + launch + synchronise (allocates the queue counter). */
 int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, int8_t* d_lattice,

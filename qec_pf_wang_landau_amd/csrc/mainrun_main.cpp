@@ -211,7 +211,8 @@ int main(int argc, char** argv)
         ok(wl_fused_launch(&plan, d_tasks.data(), d_lattice.data(), d_J.data(), d_H.data(), d_logG.data(),
                            d_factor.data(), d_energy.data(), d_off_iter.data(), d_off_lattice.data(), d_idx.data(),
                            d_cond.data(), d_cond_sample.data(), d_start.data(), d_end.data(), d_spec.data(),
-                           d_counter.data(), d_rounds.data(), d_status.data(), d_queue.data(), 50 * chunk, nullptr),
+                           d_counter.data(), d_rounds.data(), d_status.data(), d_queue.data(), nullptr, 50 * chunk,
+                           nullptr),
            "batched kernel");
         wlamd::hip_throw(hipDeviceSynchronize(), "batched kernel");
         const auto status = d_status.to_host();

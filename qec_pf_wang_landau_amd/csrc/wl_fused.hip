@@ -155,6 +155,7 @@ struct task_state {
   int64_t* counter;
   int64_t* rounds;
   int* status;
+  int64_t* ticks;  // optional: constant-rate clock ticks spent per task
 };
 
 template<typename T>
@@ -180,12 +181,11 @@ struct sweep_ctx {
   const int8_t* mask;     // spectrum flags over [E_min, E_max]
   double* G;              // walker's ln g column (stride W)
   uint32_t* H;            // walker's histogram column (stride W)
-  int X, Y, N, boundary;
+  int X, Y, N;
   int emin, erange;       // E_min, E_max - E_min
   int sk, ek, e0, shift, W, nbm1;
   bool upd;               // interval still active
   double lnf;
-  float invY;
 };
 
 // One Wang-Landau attempt (src/wl.cu:1076-1174) on LDS state, branch-free
@@ -193,12 +193,15 @@ struct sweep_ctx {
 // on the proposed energy (spectrum flag, ln g at the new bin) are issued
 // together with clamped indices, and every effect is a select or an LDS
 // atomic whose operand is zero when the effect does not apply.
+template<int BOUNDARY>
 __device__ __forceinline__ void attempt(const sweep_ctx& c, sweep_state& s, uint32_t xs, uint32_t xa)
 {
-  const int site = pick_index(xs, static_cast<uint32_t>(c.N));
-  // site / Y in FP32: (site + 1/2) / Y is at least 1/(2Y) away from an integer.
-  const int i = static_cast<int>((static_cast<float>(site) + 0.5f) * c.invY);
-  const int j = site - static_cast<int>(__umul24(static_cast<uint32_t>(i), static_cast<uint32_t>(c.Y)));
+  // site = floor(xs N / 2^32) with N = X Y; its row and column come from the
+  // 64-bit product xs X: row = high word, column = floor(low word * Y / 2^32)
+  // (xs X Y = row Y 2^32 + low Y, so floor(xs N / 2^32) = row Y + column).
+  const uint64_t px = static_cast<uint64_t>(xs) * static_cast<uint32_t>(c.X);
+  const int i = static_cast<int>(px >> 32);
+  const int j = static_cast<int>((static_cast<uint64_t>(static_cast<uint32_t>(px)) * static_cast<uint32_t>(c.Y)) >> 32);
   const int ipp = (i + 1 < c.X) ? i + 1 : 0;
   const int inn = (i > 0) ? i - 1 : c.X - 1;
   const int jpp = (j + 1 < c.Y) ? j + 1 : 0;
@@ -212,7 +215,7 @@ __device__ __forceinline__ void attempt(const sweep_ctx& c, sweep_state& s, uint
   const uint32_t ml = ((rc >> jnn) ^ (jh >> jnn)) & 1u;
   const uint32_t mr = ((rc >> jpp) ^ (jh >> j)) & 1u;
   int sum;
-  if (c.boundary == 1) {
+  if (BOUNDARY == 1) {
     const int cu = i > 0, cd = i < c.X - 1, cl = j > 0, cr = j < c.Y - 1;
     sum = cu + cd + cl + cr - 2 * static_cast<int>(cu * mu + cd * md + cl * ml + cr * mr);
   } else {
@@ -239,6 +242,37 @@ __device__ __forceinline__ void attempt(const sweep_ctx& c, sweep_state& s, uint
   c.G[at] = s.g;  // unchanged value when !upd
   atomicAdd(&c.H[at], upd ? 1u : 0u);
   s.alive = valid;
+}
+
+// n attempts of one walker starting at attempt counter att: two attempts
+// per Philox block, the next block evaluated while the current pair waits
+// on LDS.  Stops early (alive = false) at an energy outside the spectrum.
+template<int BOUNDARY>
+__device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, const attempt_stream& rng, uint64_t& att,
+                                      int n)
+{
+  int it = 0;
+  uint64_t blk = att >> 1;
+  philox4 cur = rng.eval(blk);
+  if ((att & 1u) && it < n) {  // odd start: second half of the block
+    attempt<BOUNDARY>(cx, ws, cur.v[2], cur.v[3]);
+    ++att;
+    ++it;
+    ++blk;
+    cur = rng.eval(blk);
+  }
+  for (; it + 2 <= n && ws.alive; it += 2) {
+    const philox4 nx = rng.eval(blk + 1);
+    attempt<BOUNDARY>(cx, ws, cur.v[0], cur.v[1]);
+    attempt<BOUNDARY>(cx, ws, cur.v[2], cur.v[3]);
+    att += 2;
+    ++blk;
+    cur = nx;
+  }
+  if (it < n && ws.alive) {
+    attempt<BOUNDARY>(cx, ws, cur.v[0], cur.v[1]);
+    ++att;
+  }
 }
 
 __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st, int q, unsigned char* lds,
@@ -385,7 +419,6 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
   const bool walker = tid < M;
   const int k = walker ? tid / W : 0;
   const int w = walker ? tid - k * W : 0;
-  const float invY = 1.0f / static_cast<float>(Y);
   const uint64_t c0 = walker ? st.off_iter[wbase + tid] : 0;
   uint64_t att = c0 & kAttemptMask;        // WL attempts so far
   uint64_t xdraws = c0 >> kAttemptBits;    // exchange-stream draws so far
@@ -412,32 +445,12 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       sweep_state ws{EL[tid], 0, 0.0, true};
       ws.b = (ws.e - e0) >> shift;
       ws.g = Gw[ws.b * W];
-      const sweep_ctx cx{L,  Jh, Jv, maskE0, Gw, Hw,    X,      Y,          N,   t.boundary, emin, emax - emin, sk,
-                         ek, e0, shift, W, Lnb[k] - 1, upd, lnf, invY};
-      int it = 0;
-      const int n = t.num_iterations;
-      uint64_t blk = att >> 1;
-      philox4 cur = rng.eval(blk);
-      if ((att & 1u) && it < n) {  // odd start: second half of the block
-        attempt(cx, ws, cur.v[2], cur.v[3]);
-        ++att;
-        ++it;
-        ++blk;
-        cur = rng.eval(blk);
-      }
-      // Two attempts per Philox block; the next block is evaluated while
-      // the current pair waits on LDS.
-      for (; it + 2 <= n && ws.alive; it += 2) {
-        const philox4 nx = rng.eval(blk + 1);
-        attempt(cx, ws, cur.v[0], cur.v[1]);
-        attempt(cx, ws, cur.v[2], cur.v[3]);
-        att += 2;
-        ++blk;
-        cur = nx;
-      }
-      if (it < n && ws.alive) {
-        attempt(cx, ws, cur.v[0], cur.v[1]);
-        ++att;
+      const sweep_ctx cx{L, Jh, Jv, maskE0, Gw, Hw, X, Y, N, emin, emax - emin, sk, ek, e0, shift, W, Lnb[k] - 1,
+                         upd, lnf};
+      if (t.boundary == 1) {
+        sweep<1>(cx, ws, rng, att, t.num_iterations);
+      } else {
+        sweep<0>(cx, ws, rng, att, t.num_iterations);
       }
       if (!ws.alive) {
         misc[0] = 1;
@@ -658,7 +671,11 @@ __global__ void __launch_bounds__(MAXB) k_fused(const wl_task* tasks, int num_ta
       }
       continue;
     }
+    const uint64_t t0 = wall_clock64();
     run_task(t, st, q, lds, max_rounds);
+    if (st.ticks != nullptr && threadIdx.x == 0) {
+      st.ticks[q] += static_cast<int64_t>(wall_clock64() - t0);
+    }
   }
 }
 
@@ -724,6 +741,9 @@ int wl_fused_prepare(const wl_task* h_tasks, int num_tasks, const int* h_start, 
     plan->grid = std::max(1, std::min(num_tasks, num_cu * per_cu));
     plan->resident_per_cu = per_cu;
     plan->num_cu = num_cu;
+    int khz = 0;
+    WL_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    plan->clock_khz = khz;
   });
 }
 
@@ -731,7 +751,8 @@ int wl_fused_launch(const wl_fused_launch_plan* plan, const wl_task* d_tasks, in
                     const int8_t* d_interactions, uint64_t* d_H, double* d_logG, double* d_factor, int* d_energy,
                     uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx, int8_t* d_cond, int* d_cond_sample,
                     const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
-                    int64_t* d_rounds, int* d_status, int* d_queue, int64_t max_rounds, void* stream)
+                    int64_t* d_rounds, int* d_status, int* d_queue, int64_t* d_task_ticks, int64_t max_rounds,
+                    void* stream)
 {
   return guarded([&] {
     require(plan != nullptr && d_queue != nullptr, "fused_launch: missing plan or queue counter");
@@ -740,7 +761,8 @@ int wl_fused_launch(const wl_fused_launch_plan* plan, const wl_task* d_tasks, in
     }
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     task_state s{d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy, d_offset_iter, d_offset_lattice,
-                 d_idx, d_cond, d_cond_sample, d_start, d_end, d_spectrum, d_counter, d_rounds, d_status};
+                 d_idx, d_cond, d_cond_sample, d_start, d_end, d_spectrum, d_counter, d_rounds, d_status,
+                 d_task_ticks};
     k_zero_int<<<1, 1, 0, st>>>(d_queue);
     if (plan->block <= 256) {
       k_fused<256><<<plan->grid, plan->block, plan->lds_bytes, st>>>(d_tasks, plan->num_tasks, d_queue, s,
@@ -781,7 +803,7 @@ int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, 
     WL_HIP(hipMalloc(&d_queue, sizeof(int)));
     const int rc = wl_fused_launch(&plan, d_tasks, d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy,
                                    d_offset_iter, d_offset_lattice, d_idx, d_cond, d_cond_sample, d_start, d_end,
-                                   d_spectrum, d_counter, d_rounds, d_status, d_queue, max_rounds, stream);
+                                   d_spectrum, d_counter, d_rounds, d_status, d_queue, nullptr, max_rounds, stream);
     const hipError_t sync = hipStreamSynchronize(st);
     (void)hipFree(d_queue);
     if (rc != 0) {

@@ -538,11 +538,20 @@ class FusedLauncher:
         torch = _torch()
         self.dev = dev
         b = dev.batch
-        self.plan = np.zeros(6, np.int32)
+        self.plan = np.zeros(7, np.int32)
         call("wl_fused_prepare", _p(b.tasks), b.tasks.size, _p(b.arrays["start"]), _p(b.arrays["end"]),
              _p(self.plan))
         self.queue = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.ticks = torch.zeros(b.tasks.size, dtype=torch.int64, device="cuda")
         self.pristine = {k: v.clone() for k, v in dev.t.items()}
+
+    @property
+    def clock_khz(self) -> int:
+        return int(self.plan[6])
+
+    def task_seconds(self) -> np.ndarray:
+        """Device time each task's workgroup spent, accumulated since the last restore."""
+        return to_host(self.ticks, np.int64).astype(np.float64) / (self.clock_khz * 1e3)
 
     @property
     def grid(self) -> int:
@@ -559,10 +568,11 @@ class FusedLauncher:
     def restore(self) -> None:
         for k, v in self.pristine.items():
             self.dev.t[k].copy_(v)
+        self.ticks.zero_()
 
     def launch(self, max_rounds: int = 1 << 40, stream=None) -> None:
         d = self.dev
         call("wl_fused_launch", _p(self.plan), dptr(d.d_tasks), d["lattice"], d["J"], d["H"], d["logG"],
              d["factor"], d["E"], d["off_iter"], d["off_lattice"], d["idx"], d["cond"], d["cond_sample"], d["start"],
-             d["end"], d["spectrum"], d["counter"], d["rounds"], d["status"], dptr(self.queue), max_rounds,
-             stream)
+             d["end"], d["spectrum"], d["counter"], d["rounds"], d["status"], dptr(self.queue), dptr(self.ticks),
+             max_rounds, stream)
