@@ -46,7 +46,30 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-tasks", type=int, default=16)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after timing, run one device copy of this many MiB (the reference kernel that scales "
+                         "the HBM byte counters)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "k_fused_6x6_p0.04_400tasks_traffic.txt"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the batched kernel from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same
+    counter's reading for a copy of known size), or None without one."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    vals = {}
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def build_batch(engine, samples: int):
@@ -137,6 +160,15 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = [a.elapsed_time(b) for a, b in zip(ev0, ev1)]
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: {n} bytes", file=sys.stderr, flush=True)
+        del src, dst
     result = dev.pull()
     status = result.arrays["status"]
     n_ok = int(np.sum(status == 1))
@@ -207,7 +239,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
         },
         "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None,
     }

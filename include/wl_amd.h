@@ -261,6 +261,22 @@ int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, 
                  const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
                  int64_t* d_rounds, int* d_status, int64_t max_rounds, void* stream);
 
+/* Workspace bytes wl_stitch_tasks needs for one task with I intervals whose
+ * lengths sum to total_len. */
+size_t wl_stitch_work_bytes(int num_intervals, int total_len);
+
+/* Device-side stitching (src/result_handling.cpp:368-431 without the file
+ * write) and log Z(beta[q]) for every task whose cond_sample is -1, one lane
+ * per task.  Task q writes d_out_n[q] (E, ln g) pairs at d_out_off[q] (room
+ * for end[I-1] - start[0] + 1) and d_log_z[q]; its workspace starts at byte
+ * d_work_off[q] of d_work (wl_stitch_work_bytes each, 8-byte aligned).
+ * d_out_n[q] < 0 reports a stitching failure (-2 no overlap, -3 key missing).
+ * log Z has no counterpart in the reference (README.md:198-200 cites it). */
+int wl_stitch_tasks(const wl_task* d_tasks, int num_tasks, const double* d_logG, const int* d_start, const int* d_end,
+                    const int* d_cond_sample, const double* d_beta, void* d_work, const int64_t* d_work_off,
+                    int* d_out_E, double* d_out_lng, const int64_t* d_out_off, int* d_out_n, double* d_log_z,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

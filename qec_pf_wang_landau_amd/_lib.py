@@ -49,6 +49,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "wl_fused_run": (I32, [P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, P]),
     "wl_fused_prepare": (I32, [P, I32, P, P, P]),
     "wl_fused_launch": (I32, [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, P]),
+    "wl_stitch_work_bytes": (SZ, [I32, I32]),
+    "wl_stitch_tasks": (I32, [P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P]),
     # host side (include/wl_host.h)
     "wl_interval_results": (I32, [I32, I32, U32, I32, F32, P, P, P, P]),
     "wl_energy_spectrum": (I32, [P, I32, I32, I32, P, P, P, P]),

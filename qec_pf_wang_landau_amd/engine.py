@@ -576,3 +576,54 @@ class FusedLauncher:
              d["factor"], d["E"], d["off_iter"], d["off_lattice"], d["idx"], d["cond"], d["cond_sample"], d["start"],
              d["end"], d["spectrum"], d["counter"], d["rounds"], d["status"], dptr(self.queue), dptr(self.ticks),
              max_rounds, stream)
+
+
+# ----------------------------------------------------- decode on the device --
+
+@dataclass
+class Decoded:
+    """Per task: stitched (E, ln g) and log Z; per sample: the chosen class."""
+    E: list
+    lng: list
+    log_z: np.ndarray
+    n: np.ndarray
+
+
+def stitch_tasks_gpu(dev: BatchDevice, betas: np.ndarray, stream=None) -> Decoded:
+    """wl_stitch_tasks over every task of a batch (finished tasks only)."""
+    torch = _torch()
+    b = dev.batch
+    tasks = b.tasks
+    start, end = b.arrays["start"], b.arrays["end"]
+    nq = tasks.size
+    work_off = np.zeros(nq, np.int64)
+    out_off = np.zeros(nq, np.int64)
+    w = o = 0
+    for q in range(nq):
+        i0, I = int(tasks[q]["interval_off"]), int(tasks[q]["I"])
+        total = int(np.sum(end[i0:i0 + I] - start[i0:i0 + I] + 1))
+        work_off[q] = w
+        w += int(lib().wl_stitch_work_bytes(I, total))
+        out_off[q] = o
+        o += int(end[i0 + I - 1] - start[i0] + 1)
+    work = torch.zeros(max(w, 8), dtype=torch.uint8, device="cuda")
+    keep = Keep()
+    oE = torch.zeros(max(o, 1), dtype=torch.int32, device="cuda")
+    oG = torch.zeros(max(o, 1), dtype=torch.float64, device="cuda")
+    on = torch.zeros(nq, dtype=torch.int32, device="cuda")
+    lz = torch.zeros(nq, dtype=torch.float64, device="cuda")
+    call("wl_stitch_tasks", dptr(dev.d_tasks), nq, dev["logG"], dev["start"], dev["end"], dev["cond_sample"],
+         dptr(keep(np.ascontiguousarray(betas, np.float64))), dptr(work), dptr(keep(work_off)), dptr(oE), dptr(oG),
+         dptr(keep(out_off)), dptr(on), dptr(lz), stream)
+    n = to_host(on, np.int32)
+    hE = to_host(oE, np.int32)
+    hG = to_host(oG, np.float64)
+    E = [hE[out_off[q]:out_off[q] + max(n[q], 0)].copy() for q in range(nq)]
+    G = [hG[out_off[q]:out_off[q] + max(n[q], 0)].copy() for q in range(nq)]
+    return Decoded(E, G, to_host(lz, np.float64), n)
+
+
+def decide(log_z_by_class: np.ndarray) -> np.ndarray:
+    """Most likely logical class per sample (argmax over the class axis of
+    ln Z at the Nishimori temperature); 0 = 'i' is the error's own class."""
+    return np.argmax(log_z_by_class, axis=-1)
