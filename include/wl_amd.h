@@ -261,6 +261,31 @@ int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, 
                  const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
                  int64_t* d_rounds, int* d_status, int64_t max_rounds, void* stream);
 
+/* Batched form for tasks too large for one workgroup (e.g. 16x16 with 64
+ * intervals x 64 walkers, 32x32 with 8192 walkers).  Replaces the same loop
+ * as wl_fused_launch (src/mainrun.cu:243-383, kernels of src/wl.cu:596-1175)
+ * with one workgroup per (task, interval): a block of rounds up to the next
+ * replica exchange runs with the interval's lattices (bits), ln g and
+ * histogram in LDS, then one workgroup per task runs the exchange.  The
+ * four kernels of a block are captured once into a hipGraph (flags bit 0)
+ * and replayed; the host reads one int every few blocks.  Runs every task
+ * until it finishes or max_rounds rounds have run in this call.
+ * rounds_per_block caps a block's rounds (it ends earlier at an exchange
+ * round).  Same arrays and per-task status codes as wl_fused_launch (3 is
+ * not used); bit-identical to wl_mainrun_loop and wl_fused_launch.
+ * Allocates its own scratch (one snapshot of each interval's walkers) and
+ * synchronises before returning; *blocks_out (optional) = blocks enqueued. */
+int wl_block_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, int8_t* d_lattice,
+                 const int8_t* d_interactions, uint64_t* d_H, double* d_logG, double* d_factor, int* d_energy,
+                 uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx, int8_t* d_cond, int* d_cond_sample,
+                 const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
+                 int64_t* d_rounds, int* d_status, int64_t max_rounds, int rounds_per_block, int flags,
+                 int64_t* blocks_out, void* stream);
+
+/* Largest per-interval LDS footprint of a task on the per-interval path
+ * (host interval table of task->I entries); must be <= 160 KiB. */
+size_t wl_block_interval_lds_bytes(const wl_task* task, const int* h_start, const int* h_end);
+
 /* Workspace bytes wl_stitch_tasks needs for one task with I intervals whose
  * lengths sum to total_len. */
 size_t wl_stitch_work_bytes(int num_intervals, int total_len);

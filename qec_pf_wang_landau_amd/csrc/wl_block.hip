@@ -1,0 +1,841 @@
+// Batched form for tasks too large for one workgroup (many walkers, large
+// lattices): one workgroup per (task, interval), running a block of rounds
+// between two replica exchanges with the interval's state staged in LDS.
+//
+// Between exchanges the intervals of a task do not interact: sweep,
+// flatness (src/wl.cu:892-967), averaging and redistribution
+// (src/wl.cu:710-783, 969-1031) and the factor reset (src/wl.cu:596-613) act
+// on one interval each.  Only two things couple them: the exchange at the
+// block's end (src/wl.cu:785-890), done by a per-task kernel afterwards, and
+// the sample's finish (src/wl.cu:662-708), which stops every interval in the
+// round where the last one reaches its final factor.  A block cannot know
+// that round in advance, so every interval that is final (cond = 1 after the
+// reset) keeps running, and saves a snapshot of its state at the block start
+// or at the round it became final.  When the whole task turns out final at
+// the block's end, the fix-up kernel restores every snapshot and replays
+// rounds up to the finishing round only.  This happens once per task, so
+// the common block costs one sweep kernel plus the per-task exchange.
+//
+// One block of rounds = four kernels, captured once into a hipGraph and
+// replayed; the host reads one int (tasks still running) every few blocks.
+//
+// Per interval in LDS:
+//   Jh[X], Jv[X]   coupling signs as bits, one u32 per lattice row (Y <= 32)
+//   mask[R]        the task's spectrum flags over [E_min, E_max]
+//   G[nb][W]       ln g per walker over the interval's reachable energies
+//                  e0 + stride*b, walker-minor
+//   Hc[nb][W]      histogram, u32
+//   lat[W][X]      the walkers' lattices as bits
+//   EL[W]          energies
+// Same Philox streams, draw counts and FP64 operation order as the plain
+// kernels and the one-workgroup-per-task kernel: results are bit-identical.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "wl_amd.h"
+#include "wl_common.h"
+#include "wl_sweep.h"
+
+namespace wlamd {
+namespace {
+
+// Per-interval words written by the sweep kernel for the per-task kernels.
+constexpr int kInfoWords = 4;
+constexpr int kInfoSnap = 0;   // round after which the snapshot was taken (-1 block start, -2 none)
+constexpr int kInfoFinal = 1;  // 1 if the interval is final at the block's end
+constexpr int kInfoAbort = 2;  // round in which a walker met an energy outside the spectrum (-1 none)
+constexpr int kInfoError = 3;  // bit 0: H above 32 bits, bit 1: invalid walker state
+
+struct blk_state {
+  int8_t* lattice;
+  const int8_t* J;
+  uint64_t* H;
+  double* logG;
+  double* factor;
+  int* E;
+  uint64_t* off_iter;
+  int* off_lattice;
+  int* idx;
+  int8_t* cond;
+  int* cond_sample;
+  const int* start;
+  const int* end;
+  const int8_t* spectrum;
+  int64_t* counter;
+  int64_t* rounds;
+  int* status;
+};
+
+struct blk_scratch {
+  const int* iv_task;       // global interval -> task
+  const int64_t* snap_off;  // global interval -> byte offset of its snapshot
+  unsigned char* snap;
+  int* info;                // kInfoWords per global interval
+  int* task_len;            // rounds in the current block per task (0 = idle)
+  int64_t* budget;          // rounds left in this call per task
+  int* active;              // tasks still running after the block
+  int lmax;                 // most rounds per block
+};
+
+__host__ __device__ inline int align8(int x)
+{
+  return (x + 7) & ~7;
+}
+
+__host__ __device__ inline int shift_of(int stride)
+{
+  return stride == 4 ? 2 : (stride == 2 ? 1 : 0);
+}
+
+// Bin count of an interval for reachable energies e_min + stride*m.
+__host__ __device__ inline void interval_bins(int sk, int ek, int emin, int shift, int& e0, int& nb)
+{
+  const int stride = 1 << shift;
+  const int off = sk - emin;
+  e0 = emin + ((off + stride - 1) >> shift) * stride;
+  nb = (ek >= e0) ? ((ek - e0) >> shift) + 1 : 0;
+}
+
+struct iv_plan {
+  int o_jh, o_jv, o_mask, o_G, o_H, o_lat, o_E, o_misc, bytes;
+};
+
+__host__ __device__ inline void iv_offsets(int X, int R, int nb, int W, iv_plan& p)
+{
+  int o = 0;
+  p.o_jh = o;
+  o += 4 * X;
+  p.o_jv = o;
+  o += 4 * X;
+  p.o_mask = o;
+  o = align8(o + R);
+  p.o_G = o;
+  o += 8 * nb * W;
+  p.o_H = o;
+  o += 4 * nb * W;
+  p.o_lat = o;
+  o += 4 * W * X;
+  p.o_E = o;
+  o = align8(o + 4 * W);
+  p.o_misc = o;
+  o += 32;  // 4 ints + factor
+  p.bytes = align8(o);
+}
+
+// Snapshot of a final interval: lattice bits, energies, attempt counters, ln g.
+struct snap_layout {
+  int64_t lat, E, att, G, bytes;
+};
+
+__host__ __device__ inline snap_layout snap_of(int X, int W, int nb)
+{
+  snap_layout s;
+  s.lat = 0;
+  s.E = 4LL * W * X;
+  s.att = (s.E + 4LL * W + 7) & ~7LL;
+  s.G = s.att + 8LL * W;
+  s.bytes = s.G + 8LL * nb * W;
+  return s;
+}
+
+template<typename T>
+__device__ __forceinline__ T* lds_at(unsigned char* base, int off)
+{
+  return reinterpret_cast<T*>(base + off);
+}
+
+// One interval for a block of rounds (FIX = false), or the replay of its
+// final rounds from the snapshot up to the task's finishing round (FIX =
+// true).  Workgroup = the interval's W walkers, one per lane.
+template<int MAXB, bool FIX>
+__global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_state st, blk_scratch sc)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int gi = blockIdx.x;
+  const int q = sc.iv_task[gi];
+  const int L = sc.task_len[q];
+  if (L <= 0) {
+    return;
+  }
+  const wl_task t = tasks[q];
+  const int k = gi - t.interval_off;
+  int* info = sc.info + static_cast<int64_t>(gi) * kInfoWords;
+  int r_from = 0;
+  int r_to = L - 1;
+  if (FIX) {
+    // Runs only when the whole task is final at the block's end.
+    const int* ti = sc.info + static_cast<int64_t>(t.interval_off) * kInfoWords;
+    bool fin = true;
+    int rstar = 0;
+    for (int kk = 0; kk < t.I; ++kk) {
+      const int* x = ti + kk * kInfoWords;
+      fin = fin && x[kInfoFinal] == 1 && x[kInfoAbort] < 0 && x[kInfoError] == 0;
+      rstar = max(rstar, x[kInfoSnap]);
+    }
+    if (!fin) {
+      return;
+    }
+    r_from = info[kInfoSnap] + 1;
+    r_to = rstar;
+  }
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int X = t.X, Y = t.Y, N = X * Y, W = t.W, M = t.I * t.W;
+  const int shift = shift_of(t.energy_stride);
+  const int stride = 1 << shift;
+  const int* g_start = st.start + t.interval_off;
+  const int* g_end = st.end + t.interval_off;
+  const int emin = g_start[0];
+  const int emax = g_end[t.I - 1];
+  const int R = emax - emin + 1;
+  const int sk = g_start[k], ek = g_end[k];
+  int e0, nb;
+  interval_bins(sk, ek, emin, shift, e0, nb);
+  iv_plan p;
+  iv_offsets(X, R, nb, W, p);
+  const int len0 = g_end[0] - g_start[0] + 1;
+  const int lenk = ek - sk + 1;
+  const int64_t wbase = t.walker_off + static_cast<int64_t>(k) * W;
+  const int64_t hbase = t.hist_off + static_cast<int64_t>(k) * W * len0;
+  const snap_layout sl = snap_of(X, W, nb);
+  unsigned char* snap = sc.snap + sc.snap_off[gi];
+
+  uint32_t* Jh = lds_at<uint32_t>(lds, p.o_jh);
+  uint32_t* Jv = lds_at<uint32_t>(lds, p.o_jv);
+  int8_t* mask = lds_at<int8_t>(lds, p.o_mask);
+  double* G = lds_at<double>(lds, p.o_G);
+  uint32_t* Hc = lds_at<uint32_t>(lds, p.o_H);
+  uint32_t* lat = lds_at<uint32_t>(lds, p.o_lat);
+  int* EL = lds_at<int>(lds, p.o_E);
+  int* misc = lds_at<int>(lds, p.o_misc);  // [0] new energy, [1] flat votes, [2] cond, [3] error bits
+  double* fptr = lds_at<double>(lds, p.o_misc + 16);
+
+  // ---- load --------------------------------------------------------------
+  const int8_t* Jg = st.J + t.couplings_off;
+  for (int r = tid; r < X; r += nthr) {
+    uint32_t h = 0, v = 0;
+    for (int c = 0; c < Y; ++c) {
+      h |= static_cast<uint32_t>(Jg[r * Y + c] < 0) << c;
+      v |= static_cast<uint32_t>(Jg[N + r * Y + c] < 0) << c;
+    }
+    Jh[r] = h;
+    Jv[r] = v;
+  }
+  for (int e = tid; e < R; e += nthr) {
+    mask[e] = st.spectrum[t.spectrum_off + e];
+  }
+  if (tid == 0) {
+    misc[0] = 0;
+    misc[1] = 0;
+    misc[2] = st.cond[t.interval_off + k];
+    misc[3] = 0;
+    *fptr = st.factor[wbase];
+  }
+  __syncthreads();
+  const bool walker = tid < W;
+  int slot = 0;
+  uint64_t att = 0, xhi = 0;
+  if (walker) {
+    const int64_t ol = static_cast<int64_t>(st.off_lattice[wbase + tid]) - t.lattice_off;
+    slot = static_cast<int>(ol / N);
+    const bool ok_s = ol >= 0 && ol % N == 0 && slot < M;
+    const uint64_t c0 = st.off_iter[wbase + tid];
+    xhi = c0 & ~kAttemptMask;
+    if (FIX) {
+      const uint32_t* sb = reinterpret_cast<const uint32_t*>(snap + sl.lat) + tid * X;
+      for (int r = 0; r < X; ++r) {
+        lat[tid * X + r] = sb[r];
+      }
+      EL[tid] = reinterpret_cast<const int*>(snap + sl.E)[tid];
+      att = reinterpret_cast<const uint64_t*>(snap + sl.att)[tid];
+    } else {
+      att = c0 & kAttemptMask;
+      EL[tid] = st.E[wbase + tid];
+      if (ok_s) {
+        const int8_t* src = st.lattice + t.lattice_off + static_cast<int64_t>(slot) * N;
+        for (int r = 0; r < X; ++r) {
+          uint32_t b = 0;
+          for (int c = 0; c < Y; ++c) {
+            b |= static_cast<uint32_t>(src[r * Y + c] < 0) << c;
+          }
+          lat[tid * X + r] = b;
+        }
+      }
+      const int e = EL[tid];
+      const int id = st.idx[wbase + tid];
+      const bool ok_e = e >= sk && e <= ek && ((e - emin) & (stride - 1)) == 0 && mask[e - emin] == 1;
+      const bool ok_i = id >= 0 && id < W;
+      if (!(ok_e && ok_i && ok_s)) {
+        atomicOr(&misc[3], 2);
+      }
+    }
+  }
+  // ln g and H, one (walker, bin) pair per lane-iteration, walker-major so
+  // that the HBM side is read along each walker's energies.
+  for (int x = tid; x < nb * W; x += nthr) {
+    const int w = x / nb;
+    const int b = x - w * nb;
+    const int e = e0 + b * stride - sk;
+    const int64_t oh = hbase + static_cast<int64_t>(w) * lenk + e;
+    G[b * W + w] = FIX ? reinterpret_cast<const double*>(snap + sl.G)[b * W + w] : st.logG[oh];
+    const uint64_t h = st.H[oh];
+    Hc[b * W + w] = static_cast<uint32_t>(h);
+    if (h > 0xffffffffull) {
+      atomicOr(&misc[3], 1);
+    }
+  }
+  __syncthreads();
+  if (misc[3] != 0) {
+    if (tid == 0) {
+      info[kInfoError] = misc[3];
+    }
+    return;
+  }
+
+  // ---- rounds --------------------------------------------------------------
+  attempt_stream rng;
+  rng.key0 = static_cast<uint32_t>(t.rng_key);
+  rng.slot = static_cast<uint32_t>(k * W + tid);
+  const double ebeta = wl_exp(t.beta);
+  const int8_t* maskE = mask - emin;
+  int snap_round = FIX ? info[kInfoSnap] : -2;
+  int abort_round = -1;
+
+  // Saves the state after round r (r = -1: the block's start).
+  auto save = [&](int r) {
+    if (walker) {
+      uint32_t* sb = reinterpret_cast<uint32_t*>(snap + sl.lat) + tid * X;
+      for (int rr = 0; rr < X; ++rr) {
+        sb[rr] = lat[tid * X + rr];
+      }
+      reinterpret_cast<int*>(snap + sl.E)[tid] = EL[tid];
+      reinterpret_cast<uint64_t*>(snap + sl.att)[tid] = att;
+    }
+    for (int x = tid; x < nb * W; x += nthr) {
+      reinterpret_cast<double*>(snap + sl.G)[x] = G[x];
+    }
+    snap_round = r;
+  };
+
+  if (!FIX && misc[2] == 1 && !(*fptr > ebeta)) {
+    save(-1);
+  }
+  for (int r = r_from; r <= r_to; ++r) {
+    // sweep
+    if (walker) {
+      const bool upd = misc[2] == 0;
+      const double lnf = wl_log(*fptr);
+      uint32_t* Lw = lat + tid * X;
+      sweep_state ws{EL[tid], 0, 0.0, true, 0};
+      ws.b = (ws.e - e0) >> shift;
+      ws.g = G[ws.b * W + tid];
+      const sweep_ctx cx{Lw, Jh, Jv, mask, G + tid, Hc + tid, X, Y, N, emin, emax - emin, sk, ek, e0, shift, W,
+                         nb - 1, upd, lnf};
+      if (t.boundary == 1) {
+        sweep<1>(cx, ws, rng, att, t.num_iterations);
+      } else {
+        sweep<0>(cx, ws, rng, att, t.num_iterations);
+      }
+      if (!ws.alive) {
+        misc[0] = 1;
+      }
+      EL[tid] = ws.e;
+    }
+    __syncthreads();
+    if (misc[0] != 0) {
+      abort_round = r;
+      break;
+    }
+    // flatness; cond and f are read here, before the barriers that order
+    // them against lane 0's update below
+    const int cond0 = misc[2];
+    const double f0 = *fptr;
+    if (walker && cond0 == 0) {
+      const uint32_t* Hw = Hc + tid;
+      uint32_t mn = 0xffffffffu;
+      uint64_t sum = 0;
+      int cnt = 0;
+      for (int b = 0; b < nb; ++b) {
+        if (maskE[e0 + b * stride] == 1) {
+          const uint32_t h = Hw[b * W];
+          mn = (h < mn) ? h : mn;
+          sum += h;
+          cnt += 1;
+        }
+      }
+      if (cnt > 0) {
+        const double mean = static_cast<double>(sum) / cnt;
+        if (static_cast<double>(mn) >= t.alpha * mean) {
+          atomicAdd(&misc[1], 1);
+        }
+      }
+    }
+    __syncthreads();
+    const bool flat = cond0 == 0 && misc[1] == W;
+    if (flat && walker) {
+      uint32_t* Hw = Hc + tid;
+      for (int b = 0; b < nb; ++b) {
+        Hw[b * W] = 0u;
+      }
+    }
+    const int cond1 = flat ? 1 : cond0;
+    // averaging + redistribution
+    if (cond1 == 1) {
+      const double dW = static_cast<double>(W);
+      for (int b = tid; b < nb; b += nthr) {
+        double* row = G + b * W;
+        double acc = 0.0;
+        if (maskE[e0 + b * stride] == 1) {
+          for (int ww = 0; ww < W; ++ww) {
+            acc += row[ww] / dW;
+          }
+        }
+        for (int ww = 0; ww < W; ++ww) {
+          row[ww] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    // reset (every lane computes the same values; lane 0 stores them)
+    const double f1 = flat ? wl_sqrt(f0) : f0;
+    const int cond2 = (cond1 == 1 && f1 > ebeta) ? 0 : cond1;
+    if (tid == 0) {
+      *fptr = f1;
+      misc[2] = cond2;
+      misc[1] = 0;
+    }
+    __syncthreads();
+    if (!FIX && cond2 == 1 && snap_round == -2) {
+      save(r);
+    }
+  }
+
+  // ---- write back --------------------------------------------------------
+  __syncthreads();
+  if (walker) {
+    int8_t* dst = st.lattice + t.lattice_off + static_cast<int64_t>(slot) * N;
+    for (int r = 0; r < X; ++r) {
+      const uint32_t b = lat[tid * X + r];
+      for (int c = 0; c < Y; ++c) {
+        dst[r * Y + c] = ((b >> c) & 1u) ? -1 : 1;
+      }
+    }
+    st.E[wbase + tid] = EL[tid];
+    st.off_iter[wbase + tid] = xhi | att;
+    st.factor[wbase + tid] = *fptr;
+  }
+  for (int x = tid; x < nb * W; x += nthr) {
+    const int w = x / nb;
+    const int b = x - w * nb;
+    const int64_t oh = hbase + static_cast<int64_t>(w) * lenk + (e0 + b * stride - sk);
+    st.logG[oh] = G[b * W + w];
+    st.H[oh] = Hc[b * W + w];
+  }
+  if (tid == 0) {
+    st.cond[t.interval_off + k] = static_cast<int8_t>(misc[2]);
+    if (!FIX) {
+      info[kInfoSnap] = snap_round;
+      info[kInfoFinal] = (misc[2] == 1) ? 1 : 0;
+      info[kInfoAbort] = abort_round;
+    }
+  }
+}
+
+// Block length per task: rounds up to and including the next exchange
+// round, capped by the rounds left in this call and by the block cap.
+__global__ void k_block_plan(const wl_task* tasks, int num_tasks, blk_state st, blk_scratch sc)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q == 0) {
+    *sc.active = 0;
+  }
+  if (q >= num_tasks) {
+    return;
+  }
+  const wl_task t = tasks[q];
+  int L = 0;
+  if (st.status[q] == 0 && st.cond_sample[q] != -1 && sc.budget[q] > 0) {
+    int64_t to_x = LLONG_MAX;
+    if (t.I > 1) {
+      const int64_t c = t.exchange_every;
+      int64_t m = st.counter[q] % c;
+      m += (m < 0) ? c : 0;
+      if ((t.flags & 1) != 0) {
+        to_x = (m == 0) ? 1 : LLONG_MAX;  // the compatibility counter never moves
+      } else {
+        to_x = (m == 0) ? 1 : c - m + 1;
+      }
+    }
+    L = static_cast<int>(min(min(to_x, sc.budget[q]), static_cast<int64_t>(sc.lmax)));
+  }
+  sc.task_len[q] = L;
+  for (int kk = 0; kk < t.I; ++kk) {
+    int* x = sc.info + static_cast<int64_t>(t.interval_off + kk) * kInfoWords;
+    x[kInfoSnap] = -2;
+    x[kInfoFinal] = 0;
+    x[kInfoAbort] = -1;
+    x[kInfoError] = 0;
+  }
+}
+
+// Per task after the block: finish / abort / error bookkeeping, then the
+// replica exchange (even pass, odd pass) when the block ended on an
+// exchange round.  Walker state is read and written in HBM.
+__global__ void __launch_bounds__(256) k_block_end(const wl_task* tasks, blk_state st, blk_scratch sc)
+{
+  const int q = blockIdx.x;
+  const int L = sc.task_len[q];
+  if (L <= 0) {
+    return;
+  }
+  const wl_task t = tasks[q];
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  __shared__ int s_err, s_abort, s_final, s_rstar, s_sum;
+  if (tid == 0) {
+    int err = 0, abort = INT_MAX, fin = 1, rstar = 0, sum = 0;
+    for (int kk = 0; kk < t.I; ++kk) {
+      const int* x = sc.info + static_cast<int64_t>(t.interval_off + kk) * kInfoWords;
+      err |= x[kInfoError];
+      if (x[kInfoAbort] >= 0) {
+        abort = min(abort, x[kInfoAbort]);
+      }
+      fin &= (x[kInfoFinal] == 1) ? 1 : 0;
+      rstar = max(rstar, x[kInfoSnap]);
+      sum += st.cond[t.interval_off + kk];
+    }
+    s_err = err;
+    s_abort = abort;
+    s_final = fin;
+    s_rstar = rstar;
+    s_sum = sum;
+  }
+  __syncthreads();
+  const bool compat = (t.flags & 1) != 0;
+  if (s_err != 0) {
+    if (tid == 0) {
+      st.status[q] = (s_err & 1) ? 4 : 5;
+    }
+    return;
+  }
+  if (s_abort != INT_MAX) {
+    if (tid == 0) {
+      // the aborted round counts as run but does not advance the counter
+      st.status[q] = 2;
+      st.rounds[q] += s_abort + 1;
+      if (!compat) {
+        st.counter[q] += s_abort;
+      }
+    }
+    return;
+  }
+  if (s_final != 0) {
+    // The finishing round skips the exchange but advances the counter.
+    if (tid == 0) {
+      st.status[q] = 1;
+      st.cond_sample[q] = -1;
+      st.rounds[q] += s_rstar + 1;
+      if (!compat) {
+        st.counter[q] += s_rstar + 1;
+      }
+    }
+    return;
+  }
+  const int64_t last = st.counter[q] + (compat ? 0 : L - 1);  // counter of the block's last round
+  const int W = t.W, I = t.I;
+  if (I > 1 && last % t.exchange_every == 0) {
+    const int64_t wb = t.walker_off;
+    const int* g_start = st.start + t.interval_off;
+    const int* g_end = st.end + t.interval_off;
+    const int len0 = g_end[0] - g_start[0] + 1;
+    for (int pass = 0; pass < 2; ++pass) {
+      const bool even = pass == 0;
+      for (int kk = tid; kk < I - 1; kk += nthr) {
+        const int par = (compat ? (t.parity_base + kk) : kk) & 1;
+        if (even ? par != 0 : par == 0) {
+          continue;
+        }
+        int* seg = st.idx + wb + static_cast<int64_t>(kk) * W;
+        uint64_t* oi = st.off_iter + wb + static_cast<int64_t>(kk) * W;
+        const uint64_t c0 = *oi;
+        stream_reader xr;
+        xr.init(t.rng_key, kTagExchange, static_cast<uint32_t>(kk * W), c0 >> kAttemptBits);
+        for (int ii = W - 1; ii > 0; --ii) {
+          const int r = pick_index(xr.next_u32(), static_cast<uint32_t>(ii + 1));
+          const int tmp = seg[ii];
+          seg[ii] = seg[r];
+          seg[r] = tmp;
+        }
+        *oi = (c0 & kAttemptMask) | (xr.draw << kAttemptBits);
+      }
+      __syncthreads();
+      for (int l = tid; l < (I - 1) * W; l += nthr) {
+        const int kk = l / W;
+        const int w = l - kk * W;
+        const int par = (compat ? (t.parity_base + kk) : kk) & 1;
+        if (even ? par != 0 : par == 0) {
+          continue;
+        }
+        const int ps = st.idx[wb + l];
+        const int pt = (kk + 1) * W + ps;
+        const int Et = st.E[wb + l];
+        const int Es = st.E[wb + pt];
+        const int ska = g_start[kk], eka = g_end[kk], skb = g_start[kk + 1], ekb = g_end[kk + 1];
+        if (Et <= ekb && Et >= skb && Es <= eka && Es >= ska) {
+          const double* Gi = st.logG + t.hist_off + static_cast<int64_t>(kk) * W * len0 +
+                             static_cast<int64_t>(w) * (eka - ska + 1) - ska;
+          const double* Gj = st.logG + t.hist_off + static_cast<int64_t>(kk + 1) * W * len0 +
+                             static_cast<int64_t>(ps) * (ekb - skb + 1) - skb;
+          const double giEX = Gi[Et], giEY = Gi[Es], gjEY = Gj[Es], gjEX = Gj[Et];
+          const uint64_t c0 = st.off_iter[wb + l];
+          const uint64_t xd = c0 >> kAttemptBits;
+          const uint32_t x = exchange_word(t.rng_key, static_cast<uint32_t>(l), xd);
+          st.off_iter[wb + l] = (c0 & kAttemptMask) | ((xd + 1) << kAttemptBits);
+          if (accept_exact((giEX - giEY) + (gjEY - gjEX), x)) {
+            const int so = st.off_lattice[wb + l];
+            st.off_lattice[wb + l] = st.off_lattice[wb + pt];
+            st.off_lattice[wb + pt] = so;
+            st.E[wb + l] = Es;
+            st.E[wb + pt] = Et;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    st.rounds[q] += L;
+    if (!compat) {
+      st.counter[q] += L;
+    }
+    st.cond_sample[q] = s_sum;
+    sc.budget[q] -= L;
+    if (sc.budget[q] > 0) {
+      atomicAdd(sc.active, 1);
+    }
+  }
+}
+
+__global__ void k_fill_i64(int64_t* p, int n, int64_t v)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    p[i] = v;
+  }
+}
+
+template<int MAXB>
+void launch_intervals(bool fix, int grid, int lds, hipStream_t s, const wl_task* d_tasks, const blk_state& st,
+                      const blk_scratch& sc)
+{
+  if (fix) {
+    k_interval<MAXB, true><<<grid, MAXB, lds, s>>>(d_tasks, st, sc);
+  } else {
+    k_interval<MAXB, false><<<grid, MAXB, lds, s>>>(d_tasks, st, sc);
+  }
+}
+
+}  // namespace
+}  // namespace wlamd
+
+using namespace wlamd;
+
+extern "C" {
+
+size_t wl_block_interval_lds_bytes(const wl_task* task, const int* h_start, const int* h_end)
+{
+  if (task == nullptr || h_start == nullptr || h_end == nullptr || task->I < 1 || task->W < 1) {
+    return 0;
+  }
+  const int emin = h_start[0];
+  const int R = h_end[task->I - 1] - emin + 1;
+  const int shift = shift_of(task->energy_stride);
+  size_t mx = 0;
+  for (int k = 0; k < task->I; ++k) {
+    int e0, nb;
+    interval_bins(h_start[k], h_end[k], emin, shift, e0, nb);
+    iv_plan p;
+    iv_offsets(task->X, R, nb, task->W, p);
+    mx = std::max(mx, static_cast<size_t>(p.bytes));
+  }
+  return mx;
+}
+
+int wl_block_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, int8_t* d_lattice,
+                 const int8_t* d_interactions, uint64_t* d_H, double* d_logG, double* d_factor, int* d_energy,
+                 uint64_t* d_offset_iter, int* d_offset_lattice, int* d_idx, int8_t* d_cond, int* d_cond_sample,
+                 const int* d_start, const int* d_end, const int8_t* d_spectrum, int64_t* d_counter,
+                 int64_t* d_rounds, int* d_status, int64_t max_rounds, int rounds_per_block, int flags,
+                 int64_t* blocks_out, void* stream)
+{
+  return guarded([&] {
+    require(num_tasks >= 0 && h_tasks != nullptr && d_tasks != nullptr, "block_run: bad task table");
+    require(rounds_per_block >= 1, "block_run: rounds_per_block must be >= 1");
+    if (blocks_out != nullptr) {
+      *blocks_out = 0;
+    }
+    if (num_tasks == 0 || max_rounds <= 0) {
+      return;
+    }
+    hipStream_t user = reinterpret_cast<hipStream_t>(stream);
+    int n_iv = 0;
+    for (int q = 0; q < num_tasks; ++q) {
+      const wl_task& t = h_tasks[q];
+      require(t.I >= 1 && t.W >= 1 && t.interval_off >= 0, "block_run: bad task shape");
+      n_iv = std::max(n_iv, t.interval_off + t.I);
+    }
+    std::vector<int> h_start(n_iv), h_end(n_iv);
+    WL_HIP(hipMemcpyAsync(h_start.data(), d_start, sizeof(int) * n_iv, hipMemcpyDeviceToHost, user));
+    WL_HIP(hipMemcpyAsync(h_end.data(), d_end, sizeof(int) * n_iv, hipMemcpyDeviceToHost, user));
+    WL_HIP(hipStreamSynchronize(user));
+
+    std::vector<int> iv_task(n_iv, -1);
+    std::vector<int64_t> snap_off(n_iv, 0);
+    int64_t snap_total = 0;
+    int max_w = 1;
+    int max_lds = 0;
+    for (int q = 0; q < num_tasks; ++q) {
+      const wl_task& t = h_tasks[q];
+      require(t.X >= 2 && t.Y >= 2 && t.Y <= 32, "block_run: lattice rows must be 2..32 sites wide");
+      require(t.W <= 1024, "block_run: more than 1024 walkers per interval");
+      require(t.exchange_every > 0 && t.num_iterations >= 0, "block_run: bad task parameters");
+      require(t.energy_stride == 1 || t.energy_stride == 2 || t.energy_stride == 4,
+              "block_run: energy_stride must be 1, 2 or 4");
+      const int* st = h_start.data() + t.interval_off;
+      const int* en = h_end.data() + t.interval_off;
+      const int emin = st[0];
+      const int R = en[t.I - 1] - emin + 1;
+      require(R >= 1, "block_run: empty spectrum");
+      const int shift = shift_of(t.energy_stride);
+      for (int k = 0; k < t.I; ++k) {
+        require(iv_task[t.interval_off + k] == -1, "block_run: tasks share an interval slot");
+        require(en[k] >= st[k] && st[k] >= emin && en[k] <= en[t.I - 1], "block_run: bad interval table");
+        iv_task[t.interval_off + k] = q;
+        int e0, nb;
+        interval_bins(st[k], en[k], emin, shift, e0, nb);
+        require(nb >= 1, "block_run: an interval holds no reachable energy");
+        iv_plan p;
+        iv_offsets(t.X, R, nb, t.W, p);
+        max_lds = std::max(max_lds, p.bytes);
+        snap_off[t.interval_off + k] = snap_total;
+        snap_total += (snap_of(t.X, t.W, nb).bytes + 255) & ~int64_t{255};
+      }
+      max_w = std::max(max_w, t.W);
+    }
+    for (int gi = 0; gi < n_iv; ++gi) {
+      require(iv_task[gi] >= 0, "block_run: interval slots must be covered by the tasks");
+    }
+    require(max_lds <= 160 * 1024, "block_run: an interval's state does not fit in LDS");
+    const int maxb = max_w <= 64 ? 64 : (max_w <= 256 ? 256 : 1024);
+
+    int* d_iv_task = nullptr;
+    int64_t* d_snap_off = nullptr;
+    unsigned char* d_snap = nullptr;
+    int* d_info = nullptr;
+    int* d_len = nullptr;
+    int64_t* d_budget = nullptr;
+    int* d_active = nullptr;
+    int* h_active = nullptr;
+    hipStream_t cs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&] {
+      if (exec != nullptr) {
+        (void)hipGraphExecDestroy(exec);
+      }
+      if (graph != nullptr) {
+        (void)hipGraphDestroy(graph);
+      }
+      if (cs != nullptr) {
+        (void)hipStreamSynchronize(cs);
+        (void)hipStreamDestroy(cs);
+      }
+      (void)hipFree(d_iv_task);
+      (void)hipFree(d_snap_off);
+      (void)hipFree(d_snap);
+      (void)hipFree(d_info);
+      (void)hipFree(d_len);
+      (void)hipFree(d_budget);
+      (void)hipFree(d_active);
+      (void)hipHostFree(h_active);
+    };
+    try {
+      WL_HIP(hipMalloc(&d_iv_task, sizeof(int) * n_iv));
+      WL_HIP(hipMalloc(&d_snap_off, sizeof(int64_t) * n_iv));
+      WL_HIP(hipMalloc(&d_snap, static_cast<size_t>(std::max<int64_t>(snap_total, 256))));
+      WL_HIP(hipMalloc(&d_info, sizeof(int) * kInfoWords * n_iv));
+      WL_HIP(hipMalloc(&d_len, sizeof(int) * num_tasks));
+      WL_HIP(hipMalloc(&d_budget, sizeof(int64_t) * num_tasks));
+      WL_HIP(hipMalloc(&d_active, sizeof(int)));
+      WL_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_active), sizeof(int), hipHostMallocDefault));
+      WL_HIP(hipMemcpyAsync(d_iv_task, iv_task.data(), sizeof(int) * n_iv, hipMemcpyHostToDevice, user));
+      WL_HIP(hipMemcpyAsync(d_snap_off, snap_off.data(), sizeof(int64_t) * n_iv, hipMemcpyHostToDevice, user));
+      k_fill_i64<<<grid_for(num_tasks, 256), 256, 0, user>>>(d_budget, num_tasks, max_rounds);
+      WL_LAUNCH_CHECK();
+      WL_HIP(hipStreamSynchronize(user));
+
+      blk_state s{d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy, d_offset_iter, d_offset_lattice,
+                  d_idx, d_cond, d_cond_sample, d_start, d_end, d_spectrum, d_counter, d_rounds, d_status};
+      blk_scratch sc{d_iv_task, d_snap_off, d_snap, d_info, d_len, d_budget, d_active, rounds_per_block};
+      const bool use_graph = (flags & 1) != 0;
+      // The four kernels of one block of rounds.
+      auto enqueue = [&](hipStream_t q) {
+        k_block_plan<<<grid_for(num_tasks, 256), 256, 0, q>>>(d_tasks, num_tasks, s, sc);
+        if (maxb == 64) {
+          launch_intervals<64>(false, n_iv, max_lds, q, d_tasks, s, sc);
+          launch_intervals<64>(true, n_iv, max_lds, q, d_tasks, s, sc);
+        } else if (maxb == 256) {
+          launch_intervals<256>(false, n_iv, max_lds, q, d_tasks, s, sc);
+          launch_intervals<256>(true, n_iv, max_lds, q, d_tasks, s, sc);
+        } else {
+          launch_intervals<1024>(false, n_iv, max_lds, q, d_tasks, s, sc);
+          launch_intervals<1024>(true, n_iv, max_lds, q, d_tasks, s, sc);
+        }
+        k_block_end<<<num_tasks, 256, 0, q>>>(d_tasks, s, sc);
+      };
+      hipStream_t run = user;
+      if (use_graph) {
+        WL_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        WL_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        enqueue(cs);
+        WL_HIP(hipStreamEndCapture(cs, &graph));
+        WL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        run = cs;
+      }
+      const int poll_every = 4;
+      int64_t blocks = 0;
+      for (;;) {
+        for (int i = 0; i < poll_every; ++i) {
+          if (use_graph) {
+            WL_HIP(hipGraphLaunch(exec, run));
+          } else {
+            enqueue(run);
+            WL_LAUNCH_CHECK();
+          }
+          ++blocks;
+        }
+        WL_HIP(hipMemcpyAsync(h_active, d_active, sizeof(int), hipMemcpyDeviceToHost, run));
+        WL_HIP(hipStreamSynchronize(run));
+        if (*h_active == 0) {
+          break;
+        }
+      }
+      if (blocks_out != nullptr) {
+        *blocks_out = blocks;
+      }
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
+}  // extern "C"

@@ -359,9 +359,28 @@ class BatchDevice:
              self["cond"], self["cond_sample"], self["start"], self["end"], self["spectrum"], self["counter"],
              self["rounds"], self["status"], max_rounds, None)
 
+    def run_blocks(self, max_rounds: int = 1 << 40, rounds_per_block: int = 64, graph: bool = True) -> int:
+        """wl_block_run: one workgroup per (task, interval), a block of
+        rounds per launch up to the next exchange; returns blocks enqueued."""
+        b = self.batch
+        blocks = C.c_int64(0)
+        call("wl_block_run", dptr(self.d_tasks), _p(b.tasks), b.tasks.size, self["lattice"], self["J"], self["H"],
+             self["logG"], self["factor"], self["E"], self["off_iter"], self["off_lattice"], self["idx"],
+             self["cond"], self["cond_sample"], self["start"], self["end"], self["spectrum"], self["counter"],
+             self["rounds"], self["status"], max_rounds, rounds_per_block, 1 if graph else 0, C.byref(blocks), None)
+        return blocks.value
+
     def pull(self) -> TaskBatch:
         arrays = {k: to_host(v, self.batch.arrays[k].dtype) for k, v in self.t.items()}
         return TaskBatch(self.batch.tasks, arrays, self.batch.parts)
+
+
+def block_lds_bytes(task: np.ndarray, start: np.ndarray, end: np.ndarray) -> int:
+    """Largest per-interval LDS footprint of a task on the per-interval path."""
+    tk = np.ascontiguousarray(task)
+    st = np.ascontiguousarray(start, np.int32)
+    en = np.ascontiguousarray(end, np.int32)
+    return int(lib().wl_block_interval_lds_bytes(_p(tk), _p(st), _p(en)))
 
 
 def task_lds_bytes(task: np.ndarray, start: np.ndarray, end: np.ndarray) -> int:

@@ -1,0 +1,104 @@
+"""GPU parity of the per-interval batched path (wl_block_run) against the CPU
+oracle's mainrun loop: every task's final state must be bit-identical to
+running that sample alone through the oracle, for partial runs (blocks cut
+by the round budget), runs resumed across calls, runs to convergence (the
+finishing block's snapshot replay), uneven batches, both block caps, with
+and without the captured graph, and the reference-compatible exchange
+schedule."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import compare_tasks, make_state, oracle_per_task
+from qec_pf_wang_landau_amd import engine
+
+pytestmark = pytest.mark.gpu
+
+
+def run_block(states, max_rounds=1 << 40, rounds_per_block=64, graph=True, calls=1):
+    dev = engine.BatchDevice(engine.merge_states(states))
+    for _ in range(calls):
+        dev.run_blocks(max_rounds, rounds_per_block, graph)
+    return dev.pull()
+
+
+@pytest.mark.parametrize("max_rounds", [1, 5, 33])
+@pytest.mark.parametrize("rpb", [2, 64])
+def test_block_partial_rounds_bit_exact(cuda, max_rounds, rpb):
+    st = make_state(4, 4, 0, 4, 4, 2, 0.1, "v", c=3)
+    refs = oracle_per_task(st, max_rounds)
+    got = run_block([st], max_rounds, rpb).split()[0]
+    compare_tasks(got, refs)
+    assert (got.rounds == [r.rounds[0] for r in refs]).all()
+
+
+def test_block_resume_across_calls_bit_exact(cuda):
+    st = make_state(4, 4, 1, 3, 4, 2, 0.2, "h", c=2)
+    refs = oracle_per_task(st, 40)
+    got = run_block([st], 10, calls=4).split()[0]
+    compare_tasks(got, refs)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_block_c1_to_convergence_bit_exact(cuda, graph):
+    """Runs to ln f <= 1e-6: the finishing block replays every interval from
+    its snapshot to the task's last round."""
+    st = make_state(4, 4, 0, 4, 4, 1, 0.1, "v")
+    refs = oracle_per_task(st, 1 << 40)
+    got = run_block([st], graph=graph).split()[0]
+    compare_tasks(got, refs)
+    assert got.status[0] == 1 and got.cond_sample[0] == -1
+    assert got.rounds[0] == refs[0].rounds[0]
+
+
+@pytest.mark.parametrize("rpb", [3, 64])
+def test_block_several_tasks_to_convergence_bit_exact(cuda, rpb):
+    """Tasks finishing in different blocks and at different rounds inside a
+    block (short exchange period, coarse ln f target)."""
+    st = make_state(5, 5, 1, 4, 6, 3, 0.15, "c", n=300, c=7, beta=1e-3)
+    refs = oracle_per_task(st, 1 << 40)
+    got = run_block([st], rounds_per_block=rpb).split()[0]
+    compare_tasks(got, refs)
+    assert (got.status == 1).all()
+    assert (got.rounds == [r.rounds[0] for r in refs]).all()
+
+
+def test_block_uneven_batch_bit_exact(cuda):
+    """Tasks of different sizes, boundaries, walker counts (up to two
+    wavefronts per interval) and exchange periods in one call."""
+    states = [
+        make_state(4, 4, 0, 4, 4, 2, 0.1, "i", n=300, c=4),
+        make_state(5, 5, 1, 5, 6, 1, 0.15, "v", n=400, c=3),
+        make_state(6, 6, 0, 10, 8, 2, 0.04, "c", n=500, c=5),
+        make_state(6, 4, 1, 3, 16, 1, 0.2, "h", n=200, c=1),
+        make_state(6, 6, 0, 4, 64, 1, 0.1, "v", n=100, c=6),
+        make_state(4, 6, 1, 2, 100, 1, 0.1, "i", n=50, c=2),
+    ]
+    refs = [oracle_per_task(st, 60) for st in states]
+    got = run_block(states, 60, rounds_per_block=16)
+    for g, rf in zip(got.split(), refs):
+        compare_tasks(g, rf)
+
+
+@pytest.mark.parametrize("c", [1, 3])
+def test_block_reference_exchange_schedule_bit_exact(cuda, c):
+    """Compatibility counter: exchange every round when c = 1, never
+    otherwise (the reference's loop counter stays at 1)."""
+    st = make_state(4, 4, 0, 3, 4, 1, 0.1, "v", c=c, compat=True)
+    refs = oracle_per_task(st, 25)
+    got = run_block([st], 25, rounds_per_block=8).split()[0]
+    compare_tasks(got, refs)
+
+
+def test_block_matches_fused_on_6x6_batch(cuda):
+    """Both batched forms agree on a 6x6 batch run to convergence."""
+    st = make_state(6, 6, 0, 10, 8, 3, 0.04, "v", n=1000, c=20)
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    dev.run()
+    fused = dev.pull()
+    blk = run_block([st])
+    for k, v in fused.arrays.items():
+        assert np.array_equal(v, blk.arrays[k]), k
+    assert (fused.arrays["status"] == 1).all()
