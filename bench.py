@@ -42,7 +42,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--samples", type=int, default=100)
+    ap.add_argument("--rate", type=float, default=10.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--duration", type=float, default=10.0, help="serve: arrival window (s)")
+    ap.add_argument("--hbm-gib", type=float, default=8.0, help="serve: HBM budget of the walker state (GiB)")
+    ap.add_argument("--workload", choices=("6x6", "16x16", "32x32", "serve"), default="6x6",
+                    help="6x6: the 400-task batch (default); 16x16: periodic p=0.1 samples, 64 intervals x 64 "
+                         "walkers, per-interval batched path vs the plain loop, both to ln f=1e-6; 32x32: one "
+                         "p=0.1 task, 64 intervals x 128 walkers, attempts/s over a bounded number of rounds")
+    ap.add_argument("--samples", type=int, default=None,
+                    help="disorder samples per class (default 100 for 6x6, 1 for 16x16)")
+    ap.add_argument("--rounds", type=int, default=2000, help="rounds timed by the 32x32 workload")
+    ap.add_argument("--prerun-iters", type=int, default=1000000, help="prerun steps for 16x16 and 32x32")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-tasks", type=int, default=16)
@@ -122,6 +132,178 @@ def cpu_baseline(engine, per_class, cfg, n_tasks: int):
     }
 
 
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def large_problem(engine, X: int, I: int, W: int, S: int, classes, beta: float, prerun_iters: int):
+    """Periodic p = 0.1 samples with the prerun on the GPU (4 I prerun
+    intervals so that every mainrun interval gets a seed lattice)."""
+    cfg = engine.RunConfig(X=X, Y=X, boundary=0, num_intervals=I, walker_per_interval=W, overlap=0.25, alpha=0.8,
+                           beta=beta, num_iterations=1000, exchange_every=20, seed=12345)
+    per_class = {c: engine.problem_gpu(cfg, S, 0.1, c, prerun_walkers=4 * I, prerun_iters=prerun_iters,
+                                       prerun_intervals=4 * I) for c in classes}
+    states = [engine.build_state(per_class[c], cfg) for c in classes]
+    return cfg, states
+
+
+def cpu_rounds_baseline(engine, st, cfg, rounds: int) -> dict:
+    """Oracle loop on the first task for a bounded number of rounds (one
+    thread: the walkers of one task are not split across threads)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import run_loop  # noqa: PLC0415
+    one = st.copy()
+    t0 = time.perf_counter()
+    r = run_loop(one, rounds)
+    dt = time.perf_counter() - t0
+    walkers = cfg.num_intervals * cfg.walker_per_interval * st.S
+    attempts = r * walkers * cfg.num_iterations
+    return {"value": attempts / dt, "unit": "attempts/s", "cores": 1, "kind": "port",
+            "sample": f"{r} rounds of the {st.S}-sample state ({walkers} walkers x {cfg.num_iterations} attempts "
+                      f"per round) by the oracle's single-threaded transliteration; {dt:.1f} s"}
+
+
+def bench_16x16(args, engine, torch) -> dict:
+    S = args.samples or 1
+    t_setup = time.perf_counter()
+    cfg, states = large_problem(engine, 16, 64, 64, S, engine.ERROR_CLASSES, 1e-6, args.prerun_iters)
+    log(f"16x16 setup (disorder + prerun) {time.perf_counter() - t_setup:.1f}s")
+    batch = engine.merge_states(states)
+    dev = engine.BatchDevice(batch)
+    chunk = 20000
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    blocks = 0
+    while True:
+        blocks += dev.run_blocks(chunk, 64, True)
+        status = engine.to_host(dev.t["status"], np.int32)
+        log(f"batched: {time.perf_counter() - t0:.1f}s, {int((status == 0).sum())} tasks running")
+        if (status != 0).all():
+            break
+    t_block = time.perf_counter() - t0
+    res = dev.pull()
+    rounds = res.arrays["rounds"].astype(np.int64)
+    status = res.arrays["status"]
+    ok = (status == 1).reshape(len(engine.ERROR_CLASSES), S).all(axis=0)
+    attempts = int(rounds.sum()) * 64 * 64 * cfg.num_iterations
+    log(f"batched: {t_block:.1f}s, status {np.bincount(status).tolist()}, rounds max {rounds.max()}")
+
+    plain_s = 0.0
+    plain_rounds = []
+    identical = True
+    parts = res.split()
+    log(f"batched: rounds per task {rounds.tolist()}")
+    for ci, st in enumerate(states):
+        if args.no_plain:
+            break
+        ds = engine.DeviceState(st.copy())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        total = 0
+        while True:
+            rc, r = engine.run_plain(ds, chunk)
+            total += r
+            log(f"plain class {engine.ERROR_CLASSES[ci]}: {time.perf_counter() - t1:.1f}s, {total} rounds")
+            if rc != 0 or r < chunk:
+                break
+        torch.cuda.synchronize()
+        plain_s += time.perf_counter() - t1
+        plain_rounds.append(total)
+        got = ds.pull()
+        if rc == 0:
+            identical &= bool(np.array_equal(got.logG, parts[ci].logG) and np.array_equal(got.H, parts[ci].H))
+    n_ok = int(ok.sum())
+    cpu = None if args.no_cpu else cpu_rounds_baseline(engine, states[0], cfg, 20)
+    achieved = attempts * BYTES_PER_ATTEMPT / t_block / 1e9
+    return {
+        "metric": METRIC, "value": n_ok / t_block, "unit": "samples/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": t_block * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"16x16 periodic RBIM p=0.1, {S} disorder sample(s) x 4 error classes, 64 intervals "
+                               "overlap 0.25, 64 walkers/interval, alpha 0.8, ln f 1 -> 1e-6, exchange every 20 "
+                               "rounds, 1000 attempts/walker/round; per-interval batched path (hipGraph blocks)",
+                   "tasks": int(batch.tasks.size), "samples_finished": n_ok},
+        "attempts_per_s": attempts / t_block,
+        "task_rounds": {"max": int(rounds.max()), "mean": float(rounds.mean())},
+        "blocks": blocks,
+        "plain_seconds": plain_s, "plain_rounds_per_class": plain_rounds,
+        "plain_samples_per_s": n_ok / plain_s if plain_s > 0 else None,
+        "batched_vs_plain": plain_s / t_block if plain_s > 0 else None,
+        "plain_bit_identical": identical,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_32x32(args, engine, torch) -> dict:
+    S = args.samples or 1
+    t_setup = time.perf_counter()
+    cfg, states = large_problem(engine, 32, 64, 128, S, ("i",), 1e-8, args.prerun_iters)
+    log(f"32x32 setup (disorder + prerun) {time.perf_counter() - t_setup:.1f}s")
+    dev = engine.BatchDevice(engine.merge_states(states))
+    dev.run_blocks(20, 64, True)  # warmup: first block and graph build
+    torch.cuda.synchronize()
+    dev = engine.BatchDevice(engine.merge_states(states))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.run_blocks(args.rounds, 64, True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res = dev.pull()
+    rounds = int(res.arrays["rounds"].sum())
+    attempts = rounds * 64 * 128 * cfg.num_iterations
+    log(f"32x32: {rounds} rounds in {dt:.2f}s, status {res.arrays['status'].tolist()}")
+    cpu = None if args.no_cpu else cpu_rounds_baseline(engine, states[0], cfg, 3)
+    achieved = attempts * BYTES_PER_ATTEMPT / dt / 1e9
+    return {
+        "metric": METRIC, "value": attempts / dt, "unit": "attempts/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"32x32 periodic RBIM p=0.1, {S} task(s) class i, 64 intervals overlap 0.25, 128 "
+                               f"walkers/interval (8192 walkers), {args.rounds} rounds of 1000 attempts/walker, "
+                               "exchange every 20 rounds; per-interval batched path (hipGraph blocks)",
+                   "rounds": rounds},
+        "ns_per_round": dt / max(rounds, 1) * 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_serve(args, engine, torch) -> dict:
+    from qec_pf_wang_landau_amd import serve  # noqa: PLC0415
+    reqs = serve.poisson_requests(args.rate, args.duration, seed=7)
+    srv = serve.Server(hbm_bytes=int(args.hbm_gib * (1 << 30)), slice_rounds=200)
+    log(f"serve: {len(reqs)} requests over {args.duration}s at {args.rate}/s, "
+        f"sizes {np.bincount([r.L for r in reqs]).nonzero()[0].tolist()}")
+    # warm the code paths (first prerun and launch) outside the measured stream
+    serve.Server(hbm_bytes=64 << 20, slice_rounds=200, sizes={6: serve.SIZES[6]}).serve(
+        [serve.Request(-1, 6, 0.05, seed=1)])
+    t0 = time.perf_counter()
+    res = srv.serve(reqs, t0=t0)
+    wall = time.perf_counter() - t0
+    summ = serve.latency_summary(res)
+    by_size = {}
+    for L in sorted({r.L for r in res}):
+        lat = np.array([r.latency for r in res if r.L == L and r.status == 1])
+        if lat.size:
+            by_size[str(L)] = {"n": int(lat.size), "p50_s": float(np.percentile(lat, 50)),
+                               "p99_s": float(np.percentile(lat, 99))}
+    log(f"serve: {summ} in {wall:.1f}s, {srv.slices} slices")
+    return {
+        "metric": METRIC, "value": summ.get("decoded", 0) / wall, "unit": "decodes/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"serving: Poisson arrivals at {args.rate}/s for {args.duration}s, L in 6/8/12/16 "
+                               "(weights .4/.3/.2/.1), p ~ U[0.02, 0.1], periodic, 4 error classes per request, "
+                               "(I, W) per L = " + str(serve.SIZES) + ", slices of 200 rounds, HBM budget "
+                               f"{args.hbm_gib} GiB", "requests": len(reqs)},
+        "latency": summ, "latency_by_size": by_size, "slices": srv.slices,
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -133,6 +315,11 @@ def main():
     from qec_pf_wang_landau_amd import engine  # noqa: PLC0415
 
     torch.cuda.set_device(0)
+    if args.workload != "6x6":
+        fn = {"16x16": bench_16x16, "32x32": bench_32x32, "serve": bench_serve}[args.workload]
+        print(json.dumps(fn(args, engine, torch)))
+        return
+    args.samples = args.samples or 100
     t_setup = time.perf_counter()
     cfg, per_class, states = build_batch(engine, args.samples)
     batch = engine.merge_states(states)

@@ -428,6 +428,7 @@ class DeviceState:
     def __init__(self, state: MainrunState):
         self.host = state
         self.t = {name: to_device(getattr(state, name)) for name in _STATE_ARRAYS}
+        self.counter = int(state.counter[0])  # the plain loop's exchange counter (src/mainrun.cu:241)
 
     def __getitem__(self, name):
         return dptr(self.t[name])
@@ -436,14 +437,17 @@ class DeviceState:
         out = self.host.copy()
         for name in _STATE_ARRAYS:
             setattr(out, name, to_host(self.t[name], getattr(self.host, name).dtype))
+        out.counter[:] = self.counter
         return out
 
 
 def run_plain(ds: DeviceState, max_rounds: int = 1 << 40) -> tuple[int, int]:
-    """wl_mainrun_loop over all samples of the state: (status, rounds)."""
+    """wl_mainrun_loop over all samples of the state: (status, rounds).
+    Repeated calls continue the same run (the counter is carried over); the
+    loop ended by convergence when rounds < max_rounds."""
     st = ds.host
     cfg = st.cfg
-    counter = C.c_int64(int(st.counter[0]))
+    counter = C.c_int64(ds.counter)
     rounds = C.c_int64(0)
     rc = lib().wl_mainrun_loop(
         ds["lattice"], ds["J"], ds["H"], ds["logG"], ds["factor"], ds["E"], ds["off_iter"], ds["off_lattice"],
@@ -453,6 +457,7 @@ def run_plain(ds: DeviceState, max_rounds: int = 1 << 40) -> tuple[int, int]:
         1 if cfg.compat else 0, max_rounds, C.byref(counter), C.byref(rounds), None)
     if rc not in (0, 3):
         raise _lib.WLError("wl_mainrun_loop", rc, _lib.last_error())
+    ds.counter = counter.value
     return rc, rounds.value
 
 

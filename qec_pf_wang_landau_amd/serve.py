@@ -1,0 +1,399 @@
+"""Decoding service: continuous batching of disorder-sample decoding requests
+over the batched Wang-Landau kernel, with all walker state resident in HBM.
+
+A request is one disorder sample (lattice size L, flip probability p, seed)
+to be decoded: couplings for the four error classes (src/prerun.cu:52-75),
+the spectrum prerun (src/prerun.cu:25-133), the replica-exchange
+Wang-Landau run of each class to ln f <= 1e-6 (src/mainrun.cu:243-383), the
+stitched ln g (src/result_handling.cpp:368-431), ln Z at the Nishimori
+temperature and the most likely class.
+
+Scheduling is iteration-level: the device runs slices of R rounds over every
+admitted task (one wl_fused_launch on a side stream); between slices the host
+admits waiting requests whose state fits the free HBM arenas, collects
+finished tasks and frees their memory.  A new request's couplings and prerun
+run on the default stream while a slice runs, so set-up overlaps the
+Wang-Landau work of the requests already on the device.  The kernel resumes
+each task exactly where the last slice stopped, so a request's result does
+not depend on which other requests shared its slices: it is bit-identical
+to decoding it alone.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import engine
+from ._lib import call
+
+# Lattice size -> (intervals, walkers per interval) of the serving
+# configuration: sized so that every task fits one workgroup's LDS.
+SIZES = {6: (10, 8), 8: (12, 8), 12: (16, 8), 16: (32, 8)}
+
+
+@dataclass
+class Request:
+    rid: int
+    L: int
+    p: float
+    seed: int
+    arrival: float = 0.0
+
+
+@dataclass
+class Result:
+    rid: int
+    L: int
+    log_z: np.ndarray       # [4] ln Z per class i/v/h/c
+    decision: int           # argmax class
+    lng: list               # per class: stitched ln g
+    E: list                 # per class: energies
+    rounds: np.ndarray      # per class
+    arrival: float
+    admitted: float
+    done: float
+    status: int = 1         # 1 = decoded, otherwise the first failing task status
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+class Arena:
+    """First-fit allocator of contiguous element ranges in [0, capacity)."""
+
+    def __init__(self, capacity: int):
+        self.capacity = int(capacity)
+        self.free = [(0, self.capacity)]
+
+    def alloc(self, n: int) -> int | None:
+        n = max(int(n), 1)
+        for i, (off, size) in enumerate(self.free):
+            if size >= n:
+                if size == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (off + n, size - n)
+                return off
+        return None
+
+    def release(self, off: int, n: int) -> None:
+        n = max(int(n), 1)
+        self.free.append((off, n))
+        self.free.sort()
+        merged = []
+        for o, s in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == o:
+                merged[-1] = (merged[-1][0], merged[-1][1] + s)
+            else:
+                merged.append((o, s))
+        self.free = merged
+
+    @property
+    def used(self) -> int:
+        return self.capacity - sum(s for _, s in self.free)
+
+
+# device arrays of the pool and the arena (kind) each one lives in
+_POOL = {
+    "lattice": ("lattice", np.int8), "J": ("J", np.int8), "H": ("hist", np.uint64), "logG": ("hist", np.float64),
+    "factor": ("walker", np.float64), "E": ("walker", np.int32), "off_iter": ("walker", np.uint64),
+    "off_lattice": ("walker", np.int32), "idx": ("walker", np.int32), "cond": ("interval", np.int8),
+    "start": ("interval", np.int32), "end": ("interval", np.int32), "spectrum": ("spectrum", np.int8),
+}
+_PER_TASK = ("counter", "rounds", "status", "cond_sample")
+_BYTES = {"lattice": 1, "J": 1, "hist": 16, "walker": 28, "interval": 9, "spectrum": 1}
+
+
+@dataclass
+class _Live:
+    req: Request
+    state: engine.MainrunState
+    offs: dict                 # kind -> first element
+    sizes: dict                # kind -> elements
+    tasks: np.ndarray          # 4 task descriptors with pool offsets
+    small: dict                # per-task arrays (host copy, 4 entries each)
+    admitted: float
+    done_tasks: np.ndarray = field(default_factory=lambda: np.zeros(4, bool))
+
+
+class Server:
+    """HBM-resident decoding service (one GPU)."""
+
+    def __init__(self, hbm_bytes: int = 1 << 30, slice_rounds: int = 200, max_tasks: int = 2048,
+                 prerun_walkers: int = 128, prerun_iters: int = 20000, exchange_every: int = 20,
+                 num_iterations: int = 1000, alpha: float = 0.8, beta: float = 1e-6, sizes: dict | None = None):
+        torch = engine._torch()
+        self.torch = torch
+        self.sizes = dict(SIZES if sizes is None else sizes)
+        self.slice_rounds = slice_rounds
+        self.max_tasks = max_tasks
+        self.prerun_walkers, self.prerun_iters = prerun_walkers, prerun_iters
+        self.run = dict(exchange_every=exchange_every, num_iterations=num_iterations, alpha=alpha, beta=beta)
+        # Split the HBM budget over the arenas in proportion to the largest request's needs.
+        need = self._footprint(max(self.sizes))
+        per_req = sum(need[k] * _BYTES[k] for k in need)
+        n_req = max(1, hbm_bytes // per_req)
+        self.arena = {k: Arena(need[k] * n_req) for k in need}
+        self.hbm_bytes = sum(self.arena[k].capacity * _BYTES[k] for k in self.arena)
+        self.t = {}
+        for name, (kind, dt) in _POOL.items():
+            tdt = {np.int8: torch.int8, np.int32: torch.int32, np.uint64: torch.int64, np.float64: torch.float64}[dt]
+            self.t[name] = torch.zeros(self.arena[kind].capacity, dtype=tdt, device="cuda")
+        self.h_start = np.zeros(self.arena["interval"].capacity, np.int32)
+        self.h_end = np.zeros(self.arena["interval"].capacity, np.int32)
+        # one stream and queue counter per lattice size: each size's slice is
+        # its own launch with its own LDS footprint, running concurrently
+        self.streams = {L: torch.cuda.Stream() for L in self.sizes}
+        self.queues = {L: torch.zeros(1, dtype=torch.int32, device="cuda") for L in self.sizes}
+        self.live: list[_Live] = []
+        self.waiting: list[Request] = []
+        self.results: list[Result] = []
+        self.slices = 0
+
+    # ---------------------------------------------------------------- sizes --
+    def config(self, L: int) -> engine.RunConfig:
+        I, W = self.sizes[L]
+        return engine.RunConfig(X=L, Y=L, boundary=0, num_intervals=I, walker_per_interval=W, overlap=0.25,
+                                seed=0, **self.run)
+
+    def _footprint(self, L: int) -> dict:
+        """Upper bound of a request's elements per arena (4 tasks)."""
+        I, W = self.sizes[L]
+        N = L * L
+        R = 4 * N + 1
+        lens = int(math.ceil(R / (1 + 0.25 * (I - 1)))) + 2
+        return {"lattice": 4 * I * W * N, "J": 4 * 2 * N, "hist": 4 * W * (I * (lens + 1) + lens),
+                "walker": 4 * I * W, "interval": 4 * I, "spectrum": 4 * R}
+
+    def _fits(self, need: dict) -> bool:
+        return all(max((sz for _, sz in self.arena[k].free), default=0) >= n for k, n in need.items())
+
+    # ------------------------------------------------------------ admission --
+    def submit(self, req: Request) -> None:
+        self.waiting.append(req)
+
+    def _admit(self, now: float) -> None:
+        """FIFO admission of arrived requests while the arenas have room."""
+        while self.waiting and self.waiting[0].arrival <= now and 4 * (len(self.live) + 1) <= self.max_tasks:
+            if self.waiting[0].L not in self.sizes:
+                raise ValueError(f"request {self.waiting[0].rid}: lattice size {self.waiting[0].L} not served")
+            req = self.waiting[0]
+            if not self._fits(self._footprint(req.L)):
+                return  # head of line waits for memory
+            st = self._setup(req)
+            sizes = {"lattice": st.lattice.size, "J": st.J.size, "hist": st.logG.size, "walker": st.E.size,
+                     "interval": st.start.size, "spectrum": st.spectrum.size}
+            offs = {}
+            for k, n in sizes.items():
+                o = self.arena[k].alloc(n)
+                if o is None:
+                    for kk, oo in offs.items():
+                        self.arena[kk].release(oo, sizes[kk])
+                    return  # head of line waits for memory
+                offs[k] = o
+            self.waiting.pop(0)
+            self._upload(st, offs)
+            tasks = engine.make_tasks(st)
+            tasks["walker_off"] += offs["walker"]
+            tasks["hist_off"] += offs["hist"]
+            tasks["lattice_off"] += offs["lattice"]
+            tasks["interval_off"] += offs["interval"]
+            tasks["parity_base"] += offs["interval"]
+            tasks["spectrum_off"] += offs["spectrum"]
+            tasks["couplings_off"] += offs["J"]
+            small = {k: getattr(st, k).copy() for k in _PER_TASK}
+            self.live.append(_Live(req, st, offs, sizes, tasks, small, time.perf_counter()))
+
+    def _setup(self, req: Request) -> engine.MainrunState:
+        """Couplings and prerun on the GPU, then the mainrun state of the four
+        class tasks (src/mainrun.cu:42-228); every class keys its walkers
+        with the request's seed."""
+        cfg = self.config(req.L)
+        samples = []
+        for cls in engine.ERROR_CLASSES:
+            smp = engine.problem_gpu(cfg, 1, req.p, cls, prerun_walkers=self.prerun_walkers,
+                                     prerun_iters=self.prerun_iters, prerun_intervals=max(16, 2 * cfg.num_intervals),
+                                     sample_base=req.seed)[0]
+            samples.append(smp)
+        return engine.build_state(samples, cfg)
+
+    def _upload(self, st: engine.MainrunState, offs: dict) -> None:
+        torch = self.torch
+        for name, (kind, dt) in _POOL.items():
+            a = getattr(st, name)
+            if name == "off_lattice":
+                a = (a.astype(np.int64) + offs["lattice"]).astype(np.int32)
+            src = torch.from_numpy(a.view(np.int64) if a.dtype == np.uint64 else np.ascontiguousarray(a))
+            o = offs[kind]
+            self.t[name][o:o + a.size].copy_(src)
+        o = offs["interval"]
+        self.h_start[o:o + st.start.size] = st.start
+        self.h_end[o:o + st.end.size] = st.end
+
+    def _release(self, lv: _Live) -> None:
+        for k, o in lv.offs.items():
+            self.arena[k].release(o, lv.sizes[k])
+
+    # --------------------------------------------------------------- slices --
+    def _table(self, L: int):
+        """Compact task table of the unfinished tasks of size L and their
+        small arrays."""
+        rows, small, owners = [], {k: [] for k in _PER_TASK}, []
+        for li, lv in enumerate(self.live):
+            if lv.req.L != L:
+                continue
+            for q in range(4):
+                if not lv.done_tasks[q]:
+                    rows.append(lv.tasks[q:q + 1])
+                    for k in _PER_TASK:
+                        small[k].append(lv.small[k][q:q + 1])
+                    owners.append((li, q))
+        if not rows:
+            return None, None, owners
+        return np.concatenate(rows), {k: np.concatenate(v) for k, v in small.items()}, owners
+
+    def _launch(self, L: int, tasks: np.ndarray, small: dict):
+        torch = self.torch
+        plan = np.zeros(7, np.int32)
+        call("wl_fused_prepare", engine._p(tasks), tasks.size, engine._p(self.h_start), engine._p(self.h_end),
+             engine._p(plan))
+        d_tasks = torch.from_numpy(tasks.view(np.uint8).copy()).cuda()
+        d_small = {k: engine.to_device(v) for k, v in small.items()}
+        torch.cuda.current_stream().synchronize()  # uploads complete before the slice reads them
+        d = {k: engine.dptr(v) for k, v in self.t.items()}
+        stream = self.streams[L]
+        call("wl_fused_launch", engine._p(plan), engine.dptr(d_tasks), d["lattice"], d["J"], d["H"], d["logG"],
+             d["factor"], d["E"], d["off_iter"], d["off_lattice"], d["idx"], d["cond"],
+             engine.dptr(d_small["cond_sample"]), d["start"], d["end"], d["spectrum"],
+             engine.dptr(d_small["counter"]), engine.dptr(d_small["rounds"]), engine.dptr(d_small["status"]),
+             engine.dptr(self.queues[L]), None, self.slice_rounds, C.c_void_p(stream.cuda_stream))
+        return d_tasks, d_small
+
+    def _collect(self, launched: list) -> None:
+        for L, owners, d_tasks, d_small in launched:
+            self.streams[L].synchronize()
+            small = {k: engine.to_host(v, np.int64 if k in ("counter", "rounds") else np.int32)
+                     for k, v in d_small.items()}
+            for j, (li, q) in enumerate(owners):
+                lv = self.live[li]
+                for k in _PER_TASK:
+                    lv.small[k][q] = small[k][j]
+                if small["status"][j] != 0:
+                    lv.done_tasks[q] = True
+            del d_tasks
+        finished = [lv for lv in self.live if lv.done_tasks.all()]
+        if finished:
+            self._decode(finished)
+            ids = {id(lv) for lv in finished}
+            self.live = [lv for lv in self.live if id(lv) not in ids]
+
+    def _decode(self, done: list[_Live]) -> None:
+        """Stitch + ln Z on the device for the finished requests' tasks."""
+        torch = self.torch
+        tasks = np.concatenate([lv.tasks for lv in done])
+        status = np.concatenate([lv.small["status"] for lv in done])
+        cond_sample = np.where(status == 1, -1, 0).astype(np.int32)
+        betas = np.repeat([engine.nishimori_beta(lv.req.p) for lv in done], 4)
+        h_start, h_end = self.h_start, self.h_end
+        nq = tasks.size
+        work_off = np.zeros(nq, np.int64)
+        out_off = np.zeros(nq, np.int64)
+        w = o = 0
+        for q in range(nq):
+            i0, I = int(tasks[q]["interval_off"]), int(tasks[q]["I"])
+            total = int(np.sum(h_end[i0:i0 + I] - h_start[i0:i0 + I] + 1))
+            work_off[q], out_off[q] = w, o
+            w += int(engine.lib().wl_stitch_work_bytes(I, total))
+            o += int(h_end[i0 + I - 1] - h_start[i0] + 1)
+        keep = engine.Keep()
+        work = torch.zeros(max(w, 8), dtype=torch.uint8, device="cuda")
+        oE = torch.zeros(max(o, 1), dtype=torch.int32, device="cuda")
+        oG = torch.zeros(max(o, 1), dtype=torch.float64, device="cuda")
+        on = torch.zeros(nq, dtype=torch.int32, device="cuda")
+        lz = torch.zeros(nq, dtype=torch.float64, device="cuda")
+        call("wl_stitch_tasks", engine.dptr(keep(tasks.view(np.uint8))), nq, engine.dptr(self.t["logG"]),
+             engine.dptr(self.t["start"]), engine.dptr(self.t["end"]), engine.dptr(keep(cond_sample)),
+             engine.dptr(keep(betas)), engine.dptr(work), engine.dptr(keep(work_off)), engine.dptr(oE),
+             engine.dptr(oG), engine.dptr(keep(out_off)), engine.dptr(on), engine.dptr(lz), None)
+        n = engine.to_host(on, np.int32)
+        hE, hG, hz = engine.to_host(oE, np.int32), engine.to_host(oG, np.float64), engine.to_host(lz, np.float64)
+        now = time.perf_counter()
+        for r, lv in enumerate(done):
+            qs = range(4 * r, 4 * r + 4)
+            E = [hE[out_off[q]:out_off[q] + max(n[q], 0)].copy() for q in qs]
+            G = [hG[out_off[q]:out_off[q] + max(n[q], 0)].copy() for q in qs]
+            z = hz[4 * r:4 * r + 4].copy()
+            st = lv.small["status"]
+            bad = [int(s) for s in st if s != 1]
+            self.results.append(Result(lv.req.rid, lv.req.L, z, int(np.argmax(z)), G, E, lv.small["rounds"].copy(),
+                                       lv.req.arrival, lv.admitted, now, bad[0] if bad else 1))
+            self._release(lv)
+
+    # ----------------------------------------------------------------- loop --
+    def step(self, now: float) -> bool:
+        """One slice: launch over the admitted tasks, admit arrivals while it
+        runs, collect.  Returns False when there was nothing to run."""
+        launched = []
+        for L in self.sizes:
+            tasks, small, owners = self._table(L)
+            if tasks is not None:
+                d_tasks, d_small = self._launch(L, tasks, small)
+                launched.append((L, owners, d_tasks, d_small))
+        if not launched:
+            self._admit(now)
+            return bool(self.live)
+        self._admit(now)  # overlaps the slices
+        self._collect(launched)
+        self.slices += 1
+        return True
+
+    def serve(self, requests: list[Request], t0: float | None = None, drain_s: float = 600.0) -> list[Result]:
+        """Requests arrive at t0 + arrival (seconds); runs until every one is
+        decoded or drain_s after the last arrival."""
+        t0 = time.perf_counter() if t0 is None else t0
+        pending = sorted(requests, key=lambda r: r.arrival)
+        last = pending[-1].arrival if pending else 0.0
+        for r in pending:
+            r.arrival = t0 + r.arrival
+            self.submit(r)
+        while self.waiting or self.live:
+            now = time.perf_counter()
+            if not self.step(now):
+                nxt = self.waiting[0].arrival if self.waiting else now
+                time.sleep(max(0.0, min(nxt - now, 0.01)))
+            if now - t0 > last + drain_s:
+                break
+        return self.results
+
+
+def poisson_requests(rate: float, duration: float, seed: int = 7, sizes=(6, 8, 12, 16),
+                     weights=(0.4, 0.3, 0.2, 0.1), p_range=(0.02, 0.1)) -> list[Request]:
+    """Poisson arrivals at `rate` per second over `duration` seconds; lattice
+    size drawn with `weights`, p uniform in p_range; request seed from the
+    stream seed and the request id."""
+    rng = np.random.default_rng(seed)
+    out, t, rid = [], 0.0, 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        L = int(rng.choice(sizes, p=weights))
+        p = float(rng.uniform(*p_range))
+        out.append(Request(rid, L, p, seed=1000003 * seed + 7919 * rid, arrival=t))
+        rid += 1
+    return out
+
+
+def latency_summary(results: list[Result]) -> dict:
+    lat = np.array([r.latency for r in results if r.status == 1])
+    if lat.size == 0:
+        return {"decoded": 0}
+    return {"decoded": int(lat.size), "failed": int(sum(r.status != 1 for r in results)),
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max())}

@@ -1,0 +1,59 @@
+"""Host-side pieces of the decoding service: the arena allocator and the
+request generator (no GPU)."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from qec_pf_wang_landau_amd import serve
+
+
+def test_arena_first_fit_and_coalescing():
+    a = serve.Arena(100)
+    x = a.alloc(30)
+    y = a.alloc(30)
+    z = a.alloc(30)
+    assert (x, y, z) == (0, 30, 60)
+    assert a.alloc(20) is None
+    a.release(y, 30)
+    assert a.alloc(31) is None
+    w = a.alloc(25)
+    assert w == 30
+    a.release(x, 30)
+    a.release(w, 25)
+    a.release(z, 30)
+    assert a.free == [(0, 100)] and a.used == 0
+
+
+def test_arena_random_alloc_free_never_overlaps():
+    rng = np.random.default_rng(1)
+    a = serve.Arena(1000)
+    live = {}
+    for step in range(2000):
+        if live and rng.random() < 0.5:
+            k = list(live)[rng.integers(len(live))]
+            a.release(*live.pop(k))
+        else:
+            n = int(rng.integers(1, 80))
+            o = a.alloc(n)
+            if o is not None:
+                live[step] = (o, n)
+        spans = sorted(live.values())
+        for (o1, n1), (o2, _) in zip(spans, spans[1:]):
+            assert o1 + n1 <= o2
+        assert a.used == sum(n for _, n in live.values())
+
+
+def test_poisson_requests_deterministic_and_mixed():
+    r1 = serve.poisson_requests(50.0, 4.0, seed=11)
+    r2 = serve.poisson_requests(50.0, 4.0, seed=11)
+    assert [(r.L, r.p, r.seed, r.arrival) for r in r1] == [(r.L, r.p, r.seed, r.arrival) for r in r2]
+    assert 150 < len(r1) < 260
+    arr = np.array([r.arrival for r in r1])
+    assert (np.diff(arr) > 0).all() and arr[-1] <= 4.0
+    sizes = np.array([r.L for r in r1])
+    assert set(sizes.tolist()) <= {6, 8, 12, 16}
+    assert abs(np.mean(sizes == 6) - 0.4) < 0.12
+    ps = np.array([r.p for r in r1])
+    assert ps.min() >= 0.02 and ps.max() <= 0.1
+    assert len({r.seed for r in r1}) == len(r1)
