@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--rate", type=float, default=10.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: arrival window (s)")
     ap.add_argument("--hbm-gib", type=float, default=8.0, help="serve: HBM budget of the walker state (GiB)")
-    ap.add_argument("--workload", choices=("6x6", "16x16", "32x32", "serve"), default="6x6",
+    ap.add_argument("--tasks", type=int, default=1000, help="uneven: number of tasks")
+    ap.add_argument("--workload", choices=("6x6", "16x16", "32x32", "serve", "uneven"), default="6x6",
                     help="6x6: the 400-task batch (default); 16x16: periodic p=0.1 samples, 64 intervals x 64 "
                          "walkers, per-interval batched path vs the plain loop, both to ln f=1e-6; 32x32: one "
                          "p=0.1 task, 64 intervals x 128 walkers, attempts/s over a bounded number of rounds")
@@ -304,6 +305,96 @@ def bench_serve(args, engine, torch) -> dict:
     }
 
 
+UNEVEN_IW = {8: (16, 16), 12: (32, 32), 16: (64, 64)}
+
+
+def uneven_states(engine, n_tasks: int, seed: int = 2024):
+    """1000 (sample, class) tasks: L in {8, 12, 16}, open or periodic, p ~
+    U[0.02, 0.12], class uniform; intervals x walkers per L from UNEVEN_IW.
+    Couplings per task on the GPU, the prerun batched per (L, boundary)."""
+    rng = np.random.default_rng(seed)
+    Ls = rng.choice([8, 12, 16], n_tasks)
+    bnds = rng.integers(0, 2, n_tasks)
+    ps = rng.uniform(0.02, 0.12, n_tasks)
+    cls = rng.integers(0, 4, n_tasks)
+    states = []
+    for L in (8, 12, 16):
+        for b in (0, 1):
+            ids = np.nonzero((Ls == L) & (bnds == b))[0]
+            if ids.size == 0:
+                continue
+            I, W = UNEVEN_IW[L]
+            cfg = engine.RunConfig(X=L, Y=L, boundary=int(b), num_intervals=I, walker_per_interval=W, overlap=0.25,
+                                   alpha=0.8, beta=1e-6, num_iterations=1000, exchange_every=20, seed=seed)
+            Js = [engine.disorder_gpu(seed + int(q), 1, L, L, float(ps[q]), engine.ERROR_CLASSES[cls[q]])
+                  for q in ids]
+            J = np.concatenate(Js)
+            pre = engine.prerun_gpu(J, L, L, int(b), seed + 7919 * L + b, ids.size, 4 * I, 4000 * L * L, 4 * I)
+            N = L * L
+            samples = []
+            for j, q in enumerate(ids):
+                h, fl, fe = pre[j]
+                emin, emax, mask = engine.energy_spectrum(h, L, L)
+                samples.append(engine.SampleInput(J=J[j * 2 * N:(j + 1) * 2 * N].copy(), e_min=emin, e_max=emax,
+                                                  mask=mask, found=fl, found_energy=fe, rng_key=seed + int(q)))
+            states.append(engine.build_state(samples, cfg))
+            log(f"uneven: L={L} boundary={b}: {ids.size} tasks ready")
+    return states
+
+
+def bench_uneven(args, engine, torch) -> dict:
+    t_setup = time.perf_counter()
+    states = uneven_states(engine, args.tasks)
+    log(f"uneven setup {time.perf_counter() - t_setup:.1f}s")
+    forms = engine.autotune_paths(states, rounds=20)
+    log(f"uneven autotune: { {i: (f, {k: f'{v:.3g}' for k, v in t.items()}) for i, (f, t) in forms.items()} }")
+    R = args.rounds
+    run = engine.MixedRun(states, forms)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.run(R)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    parts = run.pull()
+    attempts = sum(int(pt.rounds.sum()) * pt.cfg.num_intervals * pt.cfg.walker_per_interval * pt.cfg.num_iterations
+                   for pt in parts)
+    finished = sum(int((pt.status == 1).sum()) for pt in parts)
+    log(f"uneven: {R} rounds in {dt:.2f}s, {attempts:.3e} attempts, {finished} tasks finished")
+    plain_s = None
+    identical = None
+    if not args.no_plain:
+        plain_s = 0.0
+        identical = True
+        for st, pt in zip(states, parts):
+            ds = engine.DeviceState(st.copy())
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            engine.run_plain(ds, R)
+            torch.cuda.synchronize()
+            plain_s += time.perf_counter() - t1
+            got = ds.pull()
+            identical &= bool(np.array_equal(got.logG, pt.logG) and np.array_equal(got.lattice, pt.lattice))
+        log(f"uneven plain: {plain_s:.2f}s, identical {identical}")
+    achieved = attempts * BYTES_PER_ATTEMPT / dt / 1e9
+    return {
+        "metric": METRIC, "value": attempts / dt, "unit": "attempts/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"uneven batch: {args.tasks} tasks from seed 2024, L in 8/12/16, open or periodic, "
+                               "p ~ U[0.02, 0.12], class uniform, (I, W) = (16,16)/(32,32)/(64,64) per L, "
+                               f"{R} rounds of 1000 attempts/walker, exchange every 20; path per shape from the "
+                               "autotuner, both paths concurrently",
+                   "tasks": args.tasks, "rounds": R},
+        "paths": {f"L{st.cfg.X}_b{st.cfg.boundary}": forms[i][0] for i, st in enumerate(states)},
+        "tasks_finished": finished,
+        "plain_seconds": plain_s, "batched_vs_plain": (plain_s / dt) if plain_s else None,
+        "plain_bit_identical": identical,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -316,7 +407,7 @@ def main():
 
     torch.cuda.set_device(0)
     if args.workload != "6x6":
-        fn = {"16x16": bench_16x16, "32x32": bench_32x32, "serve": bench_serve}[args.workload]
+        fn = {"16x16": bench_16x16, "32x32": bench_32x32, "serve": bench_serve, "uneven": bench_uneven}[args.workload]
         print(json.dumps(fn(args, engine, torch)))
         return
     args.samples = args.samples or 100

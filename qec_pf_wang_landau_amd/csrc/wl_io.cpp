@@ -546,11 +546,62 @@ void write_results(const std::vector<int>& E, const std::vector<double>& lng, in
   check(H5Awrite(attr, stype, buf), "Error: Could not write attribute to dataset.");
 }
 
+std::vector<result_record> read_results(const std::string& path)
+{
+  struct entry {
+    int key;
+    double value;
+  };
+  std::vector<result_record> out;
+  h5 file(open_read(path), H5Fclose, "Error: Could not open HDF5 file:" + path);
+  for (const std::string& dname : child_names(file)) {
+    h5 dg(H5Gopen2(file, dname.c_str(), H5P_DEFAULT), H5Gclose, "disorder group");
+    for (const std::string& sname : child_names(dg)) {
+      h5 sg(H5Gopen2(dg, sname.c_str(), H5P_DEFAULT), H5Gclose, "seed group");
+      for (const std::string& tname : child_names(sg)) {
+        h5 tg(H5Gopen2(sg, tname.c_str(), H5P_DEFAULT), H5Gclose, "run group");
+        h5 ds(H5Dopen2(tg, "log_g", H5P_DEFAULT), H5Dclose, "Error: Could not open log_g dataset");
+        h5 space(H5Dget_space(ds), H5Sclose, "log_g dataspace");
+        hsize_t dims[1] = {0};
+        H5Sget_simple_extent_dims(space, dims, nullptr);
+        h5 ctype(H5Tcreate(H5T_COMPOUND, sizeof(entry)), H5Tclose, "compound type");
+        H5Tinsert(ctype, "Key", HOFFSET(entry, key), H5T_NATIVE_INT);
+        H5Tinsert(ctype, "Value", HOFFSET(entry, value), H5T_NATIVE_DOUBLE);
+        std::vector<entry> rows(dims[0]);
+        check(H5Dread(ds, ctype, H5S_ALL, H5S_ALL, H5P_DEFAULT, rows.data()), "Could not read log_g");
+        result_record r;
+        r.disorder_id = std::stoi(dname);
+        r.seed = std::stoi(sname);
+        r.timestamp = tname;
+        for (const entry& e : rows) {
+          r.E.push_back(e.key);
+          r.lng.push_back(e.value);
+        }
+        if (H5Aexists(ds, "CodeVersion") > 0) {
+          h5 attr(H5Aopen(ds, "CodeVersion", H5P_DEFAULT), H5Aclose, "CodeVersion attribute");
+          h5 stype(H5Tcopy(H5T_C_S1), H5Tclose, "string type");
+          H5Tset_size(stype, 256);
+          char buf[257] = {0};
+          check(H5Aread(attr, stype, buf), "Could not read CodeVersion");
+          r.version = buf;
+        }
+        out.push_back(std::move(r));
+      }
+    }
+  }
+  return out;
+}
+
 #else
 
 bool have_hdf5()
 {
   return false;
+}
+
+std::vector<result_record> read_results(const std::string&)
+{
+  throw std::runtime_error("built without HDF5: result files are unavailable");
 }
 
 namespace {

@@ -81,4 +81,17 @@ void read_prerun_lattices(std::map<int, std::map<int, std::vector<int8_t>>>& out
 void write_results(const std::vector<int>& E, const std::vector<double>& lng, int disorder_id, int seed,
                    const std::string& dir, const std::string& timestamp, const std::string& version);
 
+// One /{disorder}/{seed}/{timestamp}/log_g dataset of a results file.
+struct result_record {
+  int disorder_id = 0;
+  int seed = 0;
+  std::string timestamp;
+  std::vector<int> E;
+  std::vector<double> lng;
+  std::string version;
+};
+
+// Every log_g dataset of mainrun_results.h5, in group order.
+std::vector<result_record> read_results(const std::string& file);
+
 }  // namespace wlio

@@ -13,6 +13,7 @@ computation on them is a libwl_amd kernel).
 from __future__ import annotations
 
 import ctypes as C
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -651,3 +652,88 @@ def decide(log_z_by_class: np.ndarray) -> np.ndarray:
     """Most likely logical class per sample (argmax over the class axis of
     ln Z at the Nishimori temperature); 0 = 'i' is the error's own class."""
     return np.argmax(log_z_by_class, axis=-1)
+
+
+# ------------------------------------------- path choice for mixed batches --
+
+def state_fits_one_workgroup(st: MainrunState) -> bool:
+    """True when every task of the state fits the one-workgroup-per-task
+    kernel (walkers <= 1024 and the task's LDS plan <= 160 KiB)."""
+    if st.cfg.num_intervals * st.cfg.walker_per_interval > 1024:
+        return False
+    tasks = make_tasks(st)
+    I = st.cfg.num_intervals
+    for s in range(st.S):
+        if task_lds_bytes(tasks[s], st.start[s * I:(s + 1) * I], st.end[s * I:(s + 1) * I]) == 0:
+            return False
+    return True
+
+
+def autotune_paths(states: list[MainrunState], rounds: int = 40) -> dict:
+    """Times both batched forms for `rounds` rounds on copies of each state
+    that fits one workgroup and keeps the faster; states that do not fit use
+    the per-interval path.  Results are bit-identical either way, so the
+    choice only affects speed.  Returns {state index: ("fused"|"block",
+    {form: attempts/s})}."""
+    torch = _torch()
+    out = {}
+    for i, st in enumerate(states):
+        walkers = st.S * st.cfg.num_intervals * st.cfg.walker_per_interval
+        timing = {}
+        forms = ("fused", "block") if state_fits_one_workgroup(st) else ("block",)
+        for form in forms:
+            dev = BatchDevice(merge_states([st.copy()]))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if form == "fused":
+                dev.run(rounds)
+            else:
+                dev.run_blocks(rounds)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            done = int(to_host(dev.t["rounds"], np.int64).sum())
+            timing[form] = done / max(st.S, 1) * walkers * st.cfg.num_iterations / dt
+        out[i] = (max(timing, key=timing.get), timing)
+    return out
+
+
+class MixedRun:
+    """A batch of states of different shapes, each on the batched form chosen
+    for it: the one-workgroup tasks in one persistent launch on a side stream,
+    the per-interval tasks through wl_block_run at the same time."""
+
+    def __init__(self, states: list[MainrunState], forms: dict | None = None):
+        forms = forms or {i: ("fused" if state_fits_one_workgroup(st) else "block", {})
+                          for i, st in enumerate(states)}
+        self.fused_idx = [i for i in range(len(states)) if forms[i][0] == "fused"]
+        self.block_idx = [i for i in range(len(states)) if forms[i][0] == "block"]
+        self.fused = BatchDevice(merge_states([states[i] for i in self.fused_idx])) if self.fused_idx else None
+        self.block = BatchDevice(merge_states([states[i] for i in self.block_idx])) if self.block_idx else None
+
+    def run(self, max_rounds: int = 1 << 40) -> None:
+        torch = _torch()
+        side = torch.cuda.Stream()
+        queue = torch.zeros(1, dtype=torch.int32, device="cuda")
+        if self.fused is not None:
+            b = self.fused.batch
+            plan = np.zeros(7, np.int32)
+            call("wl_fused_prepare", _p(b.tasks), b.tasks.size, _p(b.arrays["start"]), _p(b.arrays["end"]),
+                 _p(plan))
+            torch.cuda.synchronize()
+            d = self.fused
+            call("wl_fused_launch", _p(plan), dptr(d.d_tasks), d["lattice"], d["J"], d["H"], d["logG"],
+                 d["factor"], d["E"], d["off_iter"], d["off_lattice"], d["idx"], d["cond"], d["cond_sample"],
+                 d["start"], d["end"], d["spectrum"], d["counter"], d["rounds"], d["status"], dptr(queue), None,
+                 max_rounds, C.c_void_p(side.cuda_stream))
+        if self.block is not None:
+            self.block.run_blocks(max_rounds)
+        side.synchronize()
+
+    def pull(self) -> list:
+        """Per input state: its TaskBatch part as a MainrunState."""
+        out = {}
+        for idx, dev in ((self.fused_idx, self.fused), (self.block_idx, self.block)):
+            if dev is not None:
+                for i, part in zip(idx, dev.pull().split()):
+                    out[i] = part
+        return [out[i] for i in sorted(out)]

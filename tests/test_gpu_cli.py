@@ -1,0 +1,78 @@
+"""The command-line programs end to end on the GPU: prerun writes
+prerun_results.h5, mainrun (plain loop and --batched) reads it and appends
+ln g to mainrun_results.h5 in the reference's layout; the records read back
+equal each other and the library path run on the same inputs (couplings,
+prerun and mainrun from the same seed, src/prerun.cu and src/mainrun.cu)."""
+
+from __future__ import annotations
+
+import subprocess
+import time
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from qec_pf_wang_landau_amd import engine
+
+pytestmark = pytest.mark.gpu
+
+BIN = Path(engine.__file__).resolve().parent / "_lib" / "bin"
+
+
+def run(args, cwd):
+    p = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=240)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return p.stdout
+
+
+def dump(path: Path):
+    recs = []
+    for line in run([str(BIN / "results_dump"), str(path)], path.parent).splitlines():
+        if line.startswith("#"):
+            _, d, s, stamp, n, *ver = line.split()
+            recs.append({"disorder": int(d), "seed": int(s), "E": [], "lng": [], "n": int(n), "ver": " ".join(ver)})
+        else:
+            e, g = line.split()
+            recs[-1]["E"].append(int(e))
+            recs[-1]["lng"].append(float(g))
+    return recs
+
+
+def test_prerun_mainrun_cli_match_library(cuda, tmp_path):
+    if not (BIN / "results_dump").exists():
+        pytest.skip("executables built without HDF5")
+    common = ["-x", "4", "-y", "4", "-p", "0.1", "-s", "12345", "-e", "v", "-r", "2", "-d", "0"]
+    run([str(BIN / "prerun"), *common, "-n", "10000", "-w", "128", "-i", "16"], tmp_path)
+    pre = list(tmp_path.rglob("prerun_results.h5"))
+    assert len(pre) == 1
+    main = ["-n", "1000", "-a", "0.8", "-b", "1e-6", "-i", "4", "-w", "4", "-o", "0.25", "-c", "20", "-f", "3600"]
+    run([str(BIN / "mainrun"), *common, *main], tmp_path)
+    time.sleep(1.1)  # run groups are named by the second (src/utils.cpp), as in the reference
+    run([str(BIN / "mainrun"), *common, *main, "--batched"], tmp_path)
+    res = list(tmp_path.rglob("mainrun_results.h5"))
+    assert len(res) == 1
+    recs = dump(res[0])
+    # two samples x two runs (plain, batched), in two timestamp groups
+    assert len(recs) == 4
+    by = {}
+    for r in recs:
+        assert r["n"] == len(r["E"]) > 0 and r["seed"] == 12345 and r["ver"]
+        by.setdefault(r["disorder"], []).append(r)
+    for d, rs in by.items():
+        assert len(rs) == 2
+        assert rs[0]["E"] == rs[1]["E"] and rs[0]["lng"] == rs[1]["lng"]
+    # the library path on the same inputs
+    cfg = engine.RunConfig(X=4, Y=4, boundary=0, num_intervals=4, walker_per_interval=4, overlap=0.25, alpha=0.8,
+                           beta=1e-6, num_iterations=1000, exchange_every=20, seed=12345)
+    samples = engine.problem_gpu(cfg, 2, 0.1, "v", prerun_walkers=128, prerun_iters=10000, prerun_intervals=16)
+    st = engine.build_state(samples, cfg)
+    ds = engine.DeviceState(st)
+    rc, _ = engine.run_plain(ds)
+    assert rc == 0
+    got = ds.pull()
+    for s in range(2):
+        a, b = got.sample_intervals(s)
+        E, lng = engine.stitch(got.sample_logG(s), a, b, cfg.walker_per_interval, cfg.X, cfg.Y)
+        assert by[s][0]["E"] == E.tolist()
+        assert np.array_equal(np.array(by[s][0]["lng"]), lng)

@@ -54,9 +54,7 @@ def test_result_alone_equals_result_in_mix(cuda):
 def test_admission_waits_for_memory(cuda):
     """A budget that holds one 6x6 request at a time still decodes all of
     them, one after another, with the same results."""
-    reqs = [r for r in requests() if r.L == 6][:2]
-    for r in reqs:
-        r.arrival = 0.0
+    reqs = [serve.Request(10, 6, 0.05, seed=901), serve.Request(11, 6, 0.08, seed=902)]
     small = serve.Server(hbm_bytes=1, slice_rounds=100, sizes={6: SIZES[6]})
     got = by_rid(small.serve([serve.Request(r.rid, r.L, r.p, r.seed, 0.0) for r in reqs]))
     roomy = serve.Server(hbm_bytes=64 << 20, slice_rounds=100, sizes={6: SIZES[6]})
