@@ -20,10 +20,13 @@
 // replayed; the host reads one int (tasks still running) every few blocks.
 //
 // Per interval in LDS:
-//   Jh[X], Jv[X]   coupling signs as bits, one u32 per lattice row (Y <= 32)
-//   mask[R]        the task's spectrum flags over [E_min, E_max]
-//   G[nb][W]       ln g per walker over the interval's reachable energies
-//                  e0 + stride*b, walker-minor
+//   JJ[X]          coupling signs as bits, {right, down} per lattice row
+//                  (Y <= 32)
+//   mask[R]        the task's spectrum flags over [E_min, E_max], and the
+//                  same flags as a bit array offset by 8 (wl_sweep.h)
+//   G[nb+2P][W]    ln g per walker over the interval's reachable energies
+//                  e0 + stride*b, walker-minor, with P = 8 >> shift zero guard
+//                  rows on both sides
 //   Hc[nb][W]      histogram, u32
 //   lat[W][X]      the walkers' lattices as bits
 //   EL[W]          energies
@@ -104,20 +107,22 @@ __host__ __device__ inline void interval_bins(int sk, int ek, int emin, int shif
 }
 
 struct iv_plan {
-  int o_jh, o_jv, o_mask, o_G, o_H, o_lat, o_E, o_misc, bytes;
+  int o_jj, o_mask, o_mbits, o_G, o_H, o_lat, o_E, o_misc, bytes, mwords, guard;
 };
 
-__host__ __device__ inline void iv_offsets(int X, int R, int nb, int W, iv_plan& p)
+__host__ __device__ inline void iv_offsets(int X, int R, int nb, int W, int shift, iv_plan& p)
 {
+  p.guard = kGuard >> shift;
+  p.mwords = ((R + 2 * kGuard) >> 5) + 2;
   int o = 0;
-  p.o_jh = o;
-  o += 4 * X;
-  p.o_jv = o;
-  o += 4 * X;
+  p.o_jj = o;
+  o += 8 * X;
   p.o_mask = o;
   o = align8(o + R);
+  p.o_mbits = o;
+  o = align8(o + 4 * p.mwords);
   p.o_G = o;
-  o += 8 * nb * W;
+  o += 8 * (nb + 2 * p.guard) * W;
   p.o_H = o;
   o += 4 * nb * W;
   p.o_lat = o;
@@ -199,7 +204,7 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
   int e0, nb;
   interval_bins(sk, ek, emin, shift, e0, nb);
   iv_plan p;
-  iv_offsets(X, R, nb, W, p);
+  iv_offsets(X, R, nb, W, shift, p);
   const int len0 = g_end[0] - g_start[0] + 1;
   const int lenk = ek - sk + 1;
   const int64_t wbase = t.walker_off + static_cast<int64_t>(k) * W;
@@ -207,10 +212,11 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
   const snap_layout sl = snap_of(X, W, nb);
   unsigned char* snap = sc.snap + sc.snap_off[gi];
 
-  uint32_t* Jh = lds_at<uint32_t>(lds, p.o_jh);
-  uint32_t* Jv = lds_at<uint32_t>(lds, p.o_jv);
+  uint2* JJ = lds_at<uint2>(lds, p.o_jj);
   int8_t* mask = lds_at<int8_t>(lds, p.o_mask);
-  double* G = lds_at<double>(lds, p.o_G);
+  uint32_t* mbits = lds_at<uint32_t>(lds, p.o_mbits);
+  double* Gall = lds_at<double>(lds, p.o_G);
+  double* G = Gall + p.guard * W;  // row 0
   uint32_t* Hc = lds_at<uint32_t>(lds, p.o_H);
   uint32_t* lat = lds_at<uint32_t>(lds, p.o_lat);
   int* EL = lds_at<int>(lds, p.o_E);
@@ -225,11 +231,25 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
       h |= static_cast<uint32_t>(Jg[r * Y + c] < 0) << c;
       v |= static_cast<uint32_t>(Jg[N + r * Y + c] < 0) << c;
     }
-    Jh[r] = h;
-    Jv[r] = v;
+    JJ[r] = make_uint2(h, v);
   }
   for (int e = tid; e < R; e += nthr) {
     mask[e] = st.spectrum[t.spectrum_off + e];
+  }
+  // bit (E - E_min + 8) of the bit array; zero outside the spectrum range
+  for (int wd = tid; wd < p.mwords; wd += nthr) {
+    uint32_t bits = 0;
+    for (int bb = 0; bb < 32; ++bb) {
+      const int e = 32 * wd + bb - kGuard;
+      if (e >= 0 && e < R && st.spectrum[t.spectrum_off + e] == 1) {
+        bits |= 1u << bb;
+      }
+    }
+    mbits[wd] = bits;
+  }
+  for (int x = tid; x < 2 * p.guard * W; x += nthr) {
+    const int row = x / W;
+    Gall[(row < p.guard ? row : nb + row) * W + (x - row * W)] = 0.0;
   }
   if (tid == 0) {
     misc[0] = 0;
@@ -336,8 +356,7 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
       sweep_state ws{EL[tid], 0, 0.0, true, 0};
       ws.b = (ws.e - e0) >> shift;
       ws.g = G[ws.b * W + tid];
-      const sweep_ctx cx{Lw, Jh, Jv, mask, G + tid, Hc + tid, X, Y, N, emin, emax - emin, sk, ek, e0, shift, W,
-                         nb - 1, upd, lnf};
+      const sweep_ctx cx{Lw, JJ, mbits, G + tid, Hc + tid, X, Y, emin, sk, ek, shift, W, upd, lnf};
       if (t.boundary == 1) {
         sweep<1>(cx, ws, rng, att, t.num_iterations);
       } else {
@@ -662,7 +681,7 @@ size_t wl_block_interval_lds_bytes(const wl_task* task, const int* h_start, cons
     int e0, nb;
     interval_bins(h_start[k], h_end[k], emin, shift, e0, nb);
     iv_plan p;
-    iv_offsets(task->X, R, nb, task->W, p);
+    iv_offsets(task->X, R, nb, task->W, shift, p);
     mx = std::max(mx, static_cast<size_t>(p.bytes));
   }
   return mx;
@@ -722,7 +741,7 @@ int wl_block_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, 
         interval_bins(st[k], en[k], emin, shift, e0, nb);
         require(nb >= 1, "block_run: an interval holds no reachable energy");
         iv_plan p;
-        iv_offsets(t.X, R, nb, t.W, p);
+        iv_offsets(t.X, R, nb, t.W, shift, p);
         max_lds = std::max(max_lds, p.bytes);
         snap_off[t.interval_off + k] = snap_total;
         snap_total += (snap_of(t.X, t.W, nb).bytes + 255) & ~int64_t{255};

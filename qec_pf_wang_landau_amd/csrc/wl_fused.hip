@@ -7,14 +7,18 @@
 // empty, so tasks of different sizes share one launch.
 //
 // LDS layout of a task (all counts from the task descriptor):
-//   Jh[X], Jv[X]          coupling signs as bits (1 = antiferromagnetic), one
-//                         u32 row per lattice row (Y <= 32)
-//   mask[R]               spectrum flags over [E_min, E_max]
+//   JJ[X]                 coupling signs as bits (1 = antiferromagnetic),
+//                         {right, down} per lattice row (Y <= 32)
+//   mask[R], mbits        spectrum flags over [E_min, E_max], bytes and the
+//                         bit array of wl_sweep.h
 //   per interval          start, end, first reachable energy e0, compact bin
-//                         count nb, bin base, cond, flat vote, factor f
-//   G[bins][W]            ln g per walker over the interval's reachable
-//                         energies only (E = e0 + stride*b), walker-minor
-//   Hc[bins][W]           histogram, u32, same indexing
+//                         count nb, G/H row base, first bin, cond, flat vote,
+//                         factor f
+//   G[rows][W]            ln g per walker over the interval's reachable
+//                         energies only (E = e0 + stride*b), walker-minor;
+//                         each interval's rows are framed by P = 8 >> shift
+//                         zero guard rows on both sides
+//   Hc[rows][W]           histogram, u32, same indexing
 //   lat[M][X]             lattices as bits (1 = spin -1), one u32 per row
 //   EL[M], slot[M], idx[M] energy, lattice slot and replica index per walker
 // Lattices never move: replica exchange swaps slots and energies, as the
@@ -44,10 +48,10 @@ namespace {
 constexpr int kMaxBlock = 1024;
 
 struct lds_plan {
-  int X, Y, N, I, W, M, R, stride_shift, total_bins;
+  int X, Y, N, I, W, M, R, stride_shift, total_bins, guard, rows, mwords;
   // byte offsets
-  int o_jh, o_jv, o_mask, o_sk, o_ek, o_e0, o_nb, o_gb, o_cond, o_vote, o_f, o_G, o_H, o_lat, o_E, o_slot, o_idx,
-    o_misc, bytes;
+  int o_jj, o_mask, o_mbits, o_sk, o_ek, o_e0, o_nb, o_gb, o_cb, o_cond, o_vote, o_f, o_G, o_H, o_lat, o_E, o_slot,
+    o_idx, o_misc, bytes;
 };
 
 // Bin count of interval k for reachable energies e_min + stride*m.
@@ -64,16 +68,19 @@ __host__ __device__ inline int align8(int x)
   return (x + 7) & ~7;
 }
 
-// Fills the byte offsets; total_bins must already be set.
+// Fills the byte offsets; total_bins and stride_shift must already be set.
 __host__ __device__ inline void plan_offsets(lds_plan& p)
 {
+  p.guard = kGuard >> p.stride_shift;
+  p.rows = p.total_bins + 2 * p.guard * p.I;
+  p.mwords = ((p.R + 2 * kGuard) >> 5) + 2;
   int o = 0;
-  p.o_jh = o;
-  o += 4 * p.X;
-  p.o_jv = o;
-  o += 4 * p.X;
+  p.o_jj = o;
+  o += 8 * p.X;
   p.o_mask = o;
   o = align8(o + p.R);
+  p.o_mbits = o;
+  o += 4 * p.mwords;
   p.o_sk = o;
   o += 4 * p.I;
   p.o_ek = o;
@@ -84,6 +91,8 @@ __host__ __device__ inline void plan_offsets(lds_plan& p)
   o += 4 * p.I;
   p.o_gb = o;
   o += 4 * p.I;
+  p.o_cb = o;
+  o += 4 * p.I;
   p.o_cond = o;
   o += 4 * p.I;
   p.o_vote = o;
@@ -91,9 +100,9 @@ __host__ __device__ inline void plan_offsets(lds_plan& p)
   p.o_f = o;
   o += 8 * p.I;
   p.o_G = o;
-  o += 8 * p.total_bins * p.W;
+  o += 8 * p.rows * p.W;
   p.o_H = o;
-  o += 4 * p.total_bins * p.W;
+  o += 4 * p.rows * p.W;
   p.o_lat = o;
   o += 4 * p.M * p.X;
   p.o_E = o;
@@ -198,9 +207,10 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
   const int shift = p.stride_shift;
   const int stride = 1 << shift;
 
-  uint32_t* Jh = lds_at<uint32_t>(lds, p.o_jh);
-  uint32_t* Jv = lds_at<uint32_t>(lds, p.o_jv);
+  uint2* JJ = lds_at<uint2>(lds, p.o_jj);
   int8_t* mask = lds_at<int8_t>(lds, p.o_mask);
+  uint32_t* mbits = lds_at<uint32_t>(lds, p.o_mbits);
+  int* Lcb = lds_at<int>(lds, p.o_cb);
   int* Lsk = lds_at<int>(lds, p.o_sk);
   int* Lek = lds_at<int>(lds, p.o_ek);
   int* Le0 = lds_at<int>(lds, p.o_e0);
@@ -229,14 +239,24 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       h |= static_cast<uint32_t>(Jg[r * Y + c] < 0) << c;
       v |= static_cast<uint32_t>(Jg[N + r * Y + c] < 0) << c;
     }
-    Jh[r] = h;
-    Jv[r] = v;
+    JJ[r] = make_uint2(h, v);
   }
   for (int e = tid; e < p.R; e += nthr) {
     mask[e] = st.spectrum[t.spectrum_off + e];
   }
+  // bit (E - E_min + 8) of the bit array; zero outside the spectrum range
+  for (int wd = tid; wd < p.mwords; wd += nthr) {
+    uint32_t bits = 0;
+    for (int bb = 0; bb < 32; ++bb) {
+      const int e = 32 * wd + bb - kGuard;
+      if (e >= 0 && e < p.R && st.spectrum[t.spectrum_off + e] == 1) {
+        bits |= 1u << bb;
+      }
+    }
+    mbits[wd] = bits;
+  }
   if (tid == 0) {
-    int gb = 0;
+    int gb = 0, cb = 0;
     for (int k = 0; k < I; ++k) {
       int e0, nb;
       interval_bins(g_start[k], g_end[k], emin, shift, e0, nb);
@@ -244,8 +264,10 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       Lek[k] = g_end[k];
       Le0[k] = e0;
       Lnb[k] = nb;
-      Lgb[k] = gb;
-      gb += nb * W;
+      Lgb[k] = gb + p.guard * W;  // row 0 of the interval, after its lower guard rows
+      Lcb[k] = cb;
+      gb += (nb + 2 * p.guard) * W;
+      cb += nb;
       Lcond[k] = st.cond[t.interval_off + k];
       Lf[k] = st.factor[wbase + static_cast<int64_t>(k) * W];
     }
@@ -274,6 +296,10 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
     const int lenk = Lek[k] - Lsk[k] + 1;
     const int64_t oh = hbase + static_cast<int64_t>(k) * W * len0 + static_cast<int64_t>(w) * lenk;
     const int nb = Lnb[k], gb = Lgb[k], e0 = Le0[k], sk = Lsk[k];
+    for (int b = -p.guard; b < 0; ++b) {  // guard rows below and above the interval
+      G[gb + b * W + w] = 0.0;
+      G[gb + (nb - 1 - b) * W + w] = 0.0;
+    }
     for (int b = 0; b < nb; ++b) {
       const int e = e0 + b * stride - sk;
       G[gb + b * W + w] = st.logG[oh + e];
@@ -322,7 +348,6 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
   int64_t rounds_done = 0;
   int status = 0;
   const int8_t* maskE = mask - emin;  // maskE[E]
-  const int8_t* maskE0 = mask;        // maskE0[E - emin]
 
   while (rounds_done < max_rounds) {
     // ---- sweep: num_iterations WL attempts per walker -----------------
@@ -336,8 +361,7 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       sweep_state ws{EL[tid], 0, 0.0, true, 0};
       ws.b = (ws.e - e0) >> shift;
       ws.g = Gw[ws.b * W];
-      const sweep_ctx cx{L, Jh, Jv, maskE0, Gw, Hw, X, Y, N, emin, emax - emin, sk, ek, e0, shift, W, Lnb[k] - 1,
-                         upd, lnf};
+      const sweep_ctx cx{L, JJ, mbits, Gw, Hw, X, Y, emin, sk, ek, shift, W, upd, lnf};
       if (t.boundary == 1) {
         sweep<1>(cx, ws, rng, att, t.num_iterations);
       } else {
@@ -398,13 +422,13 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
     for (int pr = tid; pr < p.total_bins; pr += nthr) {
       // locate the interval of compact bin pr
       int kk = 0;
-      while (kk + 1 < I && Lgb[kk + 1] <= pr * W) {
+      while (kk + 1 < I && Lcb[kk + 1] <= pr) {
         ++kk;
       }
       if (Lcond[kk] != 1) {
         continue;
       }
-      const int b = pr - Lgb[kk] / W;
+      const int b = pr - Lcb[kk];
       double* row = G + Lgb[kk] + b * W;
       double acc = 0.0;
       if (maskE[Le0[kk] + b * stride] == 1) {

@@ -1,11 +1,27 @@
 // Device code of one walker's Wang-Landau sweep on LDS state, shared by the
-// batched persistent kernel (wl_fused.hip) and the LDS-staged per-interval
-// sweep of the round-graph path (wl_plain.hip).
+// batched persistent kernel (wl_fused.hip) and the per-interval kernel of the
+// block path (wl_block.hip).
 //
-// Spins and coupling signs are bit rows (one u32 per lattice row, Y <= 32;
-// a set bit is spin -1 / an antiferromagnetic bond); ln g (f64) and H (u32)
-// are per walker over the interval's reachable energies e0 + stride*b,
-// walker-minor with stride W.
+// LDS inputs of a sweep:
+//   L[X]       the walker's spins as bit rows (one u32 per row, Y <= 32; a set
+//              bit is spin -1)
+//   JJ[X]      per row {right-bond signs, down-bond signs} as bits (a set bit
+//              is an antiferromagnetic bond), 8-byte aligned
+//   mbits[]    spectrum flags as a bit array: bit (E - E_min + 8) is set iff E
+//              is a spectrum energy; the 8 bits below E_min and every bit
+//              above E_max are zero, and one extra word follows the last
+//   G, H       ln g (f64) and the histogram (u32) of the walker's interval
+//              over its reachable energies e0 + stride*b, walker-minor (row b
+//              at b*W); G has kGuard >> shift readable guard rows below row 0
+//              and above the last row.
+//
+// One attempt issues every LDS read it needs at once: the three lattice
+// rows and two coupling rows around the site, the two spectrum words around
+// the current energy and ln g at the four bins a flip can reach (the energy
+// change of a flip is 2*s*sum over c neighbours, so it is one of +-2c,
+// +-(2c-4) or 0, whatever the spin).  The chain of an attempt is therefore
+// one LDS round trip long; the Philox block of the next pair of attempts is
+// computed in the shadow of those reads.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,6 +31,10 @@
 #include "wl_math.h"
 
 namespace wlamd {
+
+// ln g guard rows below and above an interval's bins: a flip changes the
+// energy by at most 8 = kGuard, i.e. kGuard >> shift bins.
+constexpr int kGuard = 8;
 
 // Walker registers carried across attempts.
 struct sweep_state {
@@ -27,66 +47,148 @@ struct sweep_state {
 
 // Loop-invariant inputs of one walker's sweep.
 struct sweep_ctx {
-  uint32_t* L;            // lattice rows (bits) of the walker's slot
-  const uint32_t* Jh;     // right-bond signs per row
-  const uint32_t* Jv;     // down-bond signs per row
-  const int8_t* mask;     // spectrum flags over [E_min, E_max]
-  double* G;              // walker's ln g column (stride W)
-  uint32_t* H;            // walker's histogram column (stride W)
-  int X, Y, N;
-  int emin, erange;       // E_min, E_max - E_min
-  int sk, ek, e0, shift, W, nbm1;
-  bool upd;               // interval still active
+  uint32_t* L;            // lattice rows of the walker's slot
+  const uint2* JJ;        // coupling rows
+  const uint32_t* mbits;  // spectrum bit array (see above)
+  double* G;              // walker's ln g at bin 0 (stride W)
+  uint32_t* H;            // walker's histogram at bin 0 (stride W)
+  int X, Y;
+  int emin;
+  int sk, ek, shift, W;
+  bool upd;               // interval still active: ln g and H are updated
   double lnf;
 };
 
-// One Wang-Landau attempt (src/wl.cu:1076-1174) on LDS state, branch-free
-// except for the rare exact acceptance fallback: both LDS reads that depend
-// on the proposed energy (spectrum flag, ln g at the new bin) are issued
-// together with clamped indices, and every effect is a select or an LDS
-// atomic whose operand is zero when the effect does not apply.
+// Philox4x32-10 of wl_math.h split into two halves of five rounds, so that
+// each half can fill the LDS shadow of one attempt.
+struct philox_run {
+  uint32_t c0, c1, c2, c3, k0, k1;
+
+  __device__ __forceinline__ void begin(uint64_t block, uint32_t slot, uint32_t key0)
+  {
+    c0 = static_cast<uint32_t>(block);
+    c1 = static_cast<uint32_t>(block >> 32);
+    c2 = slot;
+    c3 = 0u;
+    k0 = key0;
+    k1 = kTagWalk;
+  }
+
+  // Scheduling fence: the block state is (re)defined here, so rounds after
+  // the fence cannot move above it and rounds before it cannot sink below
+  // it; as a volatile statement it also keeps LDS accesses on their side.
+  __device__ __forceinline__ void fence() { __asm__ volatile("" : "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3)); }
+
+  __device__ __forceinline__ void rounds5()
+  {
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+      const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+      const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+      const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
+      const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+      c0 = n0;
+      c1 = static_cast<uint32_t>(p1);
+      c2 = n2;
+      c3 = static_cast<uint32_t>(p0);
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+  }
+};
+
+// Everything an attempt reads from LDS, issued before any of it is used.
+struct attempt_reads {
+  int i, j, jnn, jpp, u1, u2;
+  uint32_t rc, ru, rd, jh, jvd, jvu, m0, m1;
+  double g1, g2, g3, g4;
+};
+
 template<int BOUNDARY>
-__device__ __forceinline__ void attempt(const sweep_ctx& c, sweep_state& s, uint32_t xs, uint32_t xa)
+__device__ __forceinline__ attempt_reads attempt_issue(const sweep_ctx& c, const sweep_state& s, uint32_t xs)
 {
+  attempt_reads r;
   // site = floor(xs N / 2^32) with N = X Y; its row and column come from the
   // 64-bit product xs X: row = high word, column = floor(low word * Y / 2^32)
   // (xs X Y = row Y 2^32 + low Y, so floor(xs N / 2^32) = row Y + column).
   const uint64_t px = static_cast<uint64_t>(xs) * static_cast<uint32_t>(c.X);
-  const int i = static_cast<int>(px >> 32);
-  const int j = static_cast<int>((static_cast<uint64_t>(static_cast<uint32_t>(px)) * static_cast<uint32_t>(c.Y)) >> 32);
-  const int ipp = (i + 1 < c.X) ? i + 1 : 0;
-  const int inn = (i > 0) ? i - 1 : c.X - 1;
-  const int jpp = (j + 1 < c.Y) ? j + 1 : 0;
-  const int jnn = (j > 0) ? j - 1 : c.Y - 1;
-  const uint32_t rc = c.L[i], ru = c.L[inn], rd = c.L[ipp];
-  const uint32_t jh = c.Jh[i], jvu = c.Jv[inn], jvd = c.Jv[i];
-  const uint32_t sbit = (rc >> j) & 1u;
+  r.i = static_cast<int>(px >> 32);
+  r.j = static_cast<int>((static_cast<uint64_t>(static_cast<uint32_t>(px)) * static_cast<uint32_t>(c.Y)) >> 32);
+  const int ipp = (r.i + 1 < c.X) ? r.i + 1 : 0;
+  const int inn = (r.i > 0) ? r.i - 1 : c.X - 1;
+  r.jpp = (r.j + 1 < c.Y) ? r.j + 1 : 0;
+  r.jnn = (r.j > 0) ? r.j - 1 : c.Y - 1;
+  // reachable energy changes: +-2nc and +-(2nc - 4) for nc neighbours
+  int nc = 4;
+  if (BOUNDARY == 1) {
+    nc = (r.i > 0) + (r.i < c.X - 1) + (r.j > 0) + (r.j < c.Y - 1);
+  }
+  r.u1 = 2 * nc;
+  r.u2 = 2 * nc - 4;
+  r.rc = c.L[r.i];
+  r.ru = c.L[inn];
+  r.rd = c.L[ipp];
+  const uint2 jj = c.JJ[r.i];
+  r.jh = jj.x;
+  r.jvd = jj.y;
+  r.jvu = c.JJ[inn].y;
+  const int rel = s.e - c.emin;  // >= 0: the current energy is a spectrum energy
+  r.m0 = c.mbits[rel >> 5];
+  r.m1 = c.mbits[(rel >> 5) + 1];
+  const double* Gb = c.G + __mul24(s.b, c.W);
+  r.g1 = Gb[__mul24(r.u1 >> c.shift, c.W)];
+  r.g2 = Gb[__mul24(r.u2 >> c.shift, c.W)];
+  r.g3 = Gb[-__mul24(r.u2 >> c.shift, c.W)];
+  r.g4 = Gb[-__mul24(r.u1 >> c.shift, c.W)];
+  return r;
+}
+
+// The rest of one Wang-Landau attempt (src/wl.cu:1076-1174) once its reads
+// have landed: branch-free except for the rare exact acceptance fallback;
+// every effect is a select or an LDS atomic whose operand is zero when the
+// effect does not apply.
+template<int BOUNDARY>
+__device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& s, const attempt_reads& r, uint32_t xa)
+{
+  const int j = r.j;
+  const uint32_t sbit = (r.rc >> j) & 1u;
   // mismatch (s_nb J_nb = -1) per neighbour
-  const uint32_t mu = ((ru ^ jvu) >> j) & 1u;
-  const uint32_t md = ((rd ^ jvd) >> j) & 1u;
-  const uint32_t ml = ((rc >> jnn) ^ (jh >> jnn)) & 1u;
-  const uint32_t mr = ((rc >> jpp) ^ (jh >> j)) & 1u;
+  const uint32_t mu = ((r.ru ^ r.jvu) >> j) & 1u;
+  const uint32_t md = ((r.rd ^ r.jvd) >> j) & 1u;
+  const uint32_t ml = ((r.rc >> r.jnn) ^ (r.jh >> r.jnn)) & 1u;
+  const uint32_t mr = ((r.rc >> r.jpp) ^ (r.jh >> j)) & 1u;
   int sum;
   if (BOUNDARY == 1) {
-    const int cu = i > 0, cd = i < c.X - 1, cl = j > 0, cr = j < c.Y - 1;
+    const int cu = r.i > 0, cd = r.i < c.X - 1, cl = j > 0, cr = j < c.Y - 1;
     sum = cu + cd + cl + cr - 2 * static_cast<int>(cu * mu + cd * md + cl * ml + cr * mr);
   } else {
     sum = 4 - 2 * static_cast<int>(mu + md + ml + mr);
   }
-  const int neg = -static_cast<int>(sbit);              // 0 or -1
-  const int en = s.e + (((2 * sum) ^ neg) - neg);        // E + 2 s sum, s = +1/-1
-  const int rel = en - c.emin;
-  const bool in_spec_range = static_cast<uint32_t>(rel) <= static_cast<uint32_t>(c.erange);
-  const bool in_range = (en >= c.sk) & (en <= c.ek);
-  // clamped so that the speculative read stays inside the walker's column
-  const int b_new = in_range ? min(max((en - c.e0) >> c.shift, 0), c.nbm1) : s.b;
-  const int8_t flag = c.mask[in_spec_range ? rel : 0];
-  const double g_new = c.G[__mul24(b_new, c.W)];
-  const bool valid = s.alive & in_spec_range & (flag == 1);
+  const int neg = -static_cast<int>(sbit);   // 0 or -1
+  const int de = ((2 * sum) ^ neg) - neg;     // 2 s sum, s = +1/-1
+  const int en = s.e + de;
+  // flags of E - 8 .. E + 23 from the two words around the current energy
+  const int rel = s.e - c.emin;
+  const uint32_t win =
+    static_cast<uint32_t>(((static_cast<uint64_t>(r.m1) << 32) | r.m0) >> static_cast<uint32_t>(rel & 31));
+  const bool flag = ((win >> (de + kGuard)) & 1u) != 0;
+  const bool in_range = static_cast<uint32_t>(en - c.sk) <= static_cast<uint32_t>(c.ek - c.sk);
+  // The four candidates stay four loads issued up front: without this empty
+  // barrier the compiler folds "select of loads" into one load of a selected
+  // address, which puts a second LDS round trip (and branches) back on the
+  // attempt's chain.
+  double g1 = r.g1, g2 = r.g2, g3 = r.g3, g4 = r.g4;
+  __asm__ volatile("" : "+v"(g1), "+v"(g2), "+v"(g3), "+v"(g4));
+  double g_new = s.g;
+  g_new = (de == -r.u1) ? g4 : g_new;
+  g_new = (de == -r.u2) ? g3 : g_new;
+  g_new = (de == r.u2) ? g2 : g_new;
+  g_new = (de == r.u1) ? g1 : g_new;
+  const bool valid = s.alive & flag;
   const bool acc = valid & in_range & accept_fast(s.g - g_new, xa);
-  atomicXor(&c.L[i], acc ? (1u << j) : 0u);
+  atomicXor(&c.L[r.i], acc ? (1u << j) : 0u);
   s.e = acc ? en : s.e;
-  s.b = acc ? b_new : s.b;
+  s.b = acc ? s.b + (de >> c.shift) : s.b;
   s.g = acc ? g_new : s.g;
   const bool upd = valid & c.upd;
   s.g = upd ? s.g + c.lnf : s.g;
@@ -98,9 +200,10 @@ __device__ __forceinline__ void attempt(const sweep_ctx& c, sweep_state& s, uint
 }
 
 // n attempts of one walker starting at attempt counter att: two attempts
-// per Philox block, the next block evaluated while the current pair waits
-// on LDS.  Stops early (alive = false) at an energy outside the spectrum;
-// att then counts the attempts up to and including that one.
+// per Philox block; the block of the next pair is computed, five rounds at
+// a time, while each attempt of the current pair waits for its reads.
+// Stops early (alive = false) at an energy outside the spectrum; att then
+// counts the attempts up to and including that one.
 template<int BOUNDARY>
 __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, const attempt_stream& rng, uint64_t& att,
                                       int n)
@@ -109,23 +212,37 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
   uint64_t blk = att >> 1;
   philox4 cur = rng.eval(blk);
   if ((att & 1u) && it < n) {  // odd start: second half of the block
-    attempt<BOUNDARY>(cx, ws, cur.v[2], cur.v[3]);
+    const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
+    attempt_finish<BOUNDARY>(cx, ws, r, cur.v[3]);
     ++att;
     ++it;
     ++blk;
     cur = rng.eval(blk);
   }
   for (; it + 2 <= n && ws.alive; it += 2) {
-    const philox4 nx = rng.eval(blk + 1);
-    attempt<BOUNDARY>(cx, ws, cur.v[0], cur.v[1]);
+    // [reads of attempt 1] five rounds [rest of attempt 1] [reads of
+    // attempt 2] five rounds [rest of attempt 2]: the fences keep each half
+    // of the next block between an attempt's reads and their first use.
+    philox_run nx;
+    nx.begin(blk + 1, rng.slot, rng.key0);
+    const attempt_reads r1 = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
+    nx.fence();
+    nx.rounds5();
+    nx.fence();
+    attempt_finish<BOUNDARY>(cx, ws, r1, cur.v[1]);
     const bool first_ok = ws.alive;
-    attempt<BOUNDARY>(cx, ws, cur.v[2], cur.v[3]);  // no effect once dead
+    const attempt_reads r2 = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
+    nx.fence();
+    nx.rounds5();
+    nx.fence();
+    attempt_finish<BOUNDARY>(cx, ws, r2, cur.v[3]);  // no effect once dead
     att += first_ok ? 2 : 1;  // the attempt that met the new energy counts, later ones do not
     ++blk;
-    cur = nx;
+    cur = philox4{{nx.c0, nx.c1, nx.c2, nx.c3}};
   }
   if (it < n && ws.alive) {
-    attempt<BOUNDARY>(cx, ws, cur.v[0], cur.v[1]);
+    const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
+    attempt_finish<BOUNDARY>(cx, ws, r, cur.v[1]);
     ++att;
   }
 }
