@@ -137,6 +137,21 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def copy_probe(args, torch) -> None:
+    """One device copy of --copy-probe-mib MiB after the timed work: a kernel
+    of known traffic that scales the HBM byte counters of a profile run."""
+    if args.copy_probe_mib <= 0:
+        return
+    n = args.copy_probe_mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: {n} bytes")
+    del src, dst
+
+
 def large_problem(engine, X: int, I: int, W: int, S: int, classes, beta: float, prerun_iters: int):
     """Periodic p = 0.1 samples with the prerun on the GPU (4 I prerun
     intervals so that every mainrun interval gets a seed lattice)."""
@@ -255,6 +270,7 @@ def bench_32x32(args, engine, torch) -> dict:
     rounds = int(res.arrays["rounds"].sum())
     attempts = rounds * 64 * 128 * cfg.num_iterations
     log(f"32x32: {rounds} rounds in {dt:.2f}s, status {res.arrays['status'].tolist()}")
+    copy_probe(args, torch)
     cpu = None if args.no_cpu else cpu_rounds_baseline(engine, states[0], cfg, 3)
     achieved = attempts * BYTES_PER_ATTEMPT / dt / 1e9
     return {
@@ -438,15 +454,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = [a.elapsed_time(b) for a, b in zip(ev0, ev1)]
-    if args.copy_probe_mib > 0:
-        n = args.copy_probe_mib << 20
-        src = torch.ones(n, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"[bench] copy probe: {n} bytes", file=sys.stderr, flush=True)
-        del src, dst
+    copy_probe(args, torch)
     result = dev.pull()
     status = result.arrays["status"]
     n_ok = int(np.sum(status == 1))

@@ -352,32 +352,47 @@ WL_HD bool accept_exact(double d, uint32_t x)
 }
 
 #if defined(__HIPCC__) || defined(__HIP__)
-// Same decision as accept_exact, faster on the device: exp(d) is first
-// estimated in FP32 with the hardware exp2 and u in FP32; the FP64 comparison
-// is only made when u lies within a relative 2^-12 band around the estimate.
-// The estimate is within 2^-16 relative of exp(d) for -80 <= d < 0 (FP32
-// rounding of d log2 e contributes < 2^-17 for |d log2 e| < 128, v_exp_f32
-// 1 ulp) and the FP32 u within 2^-23 of u, so outside the band both
-// comparisons agree; below -80, exp(d) < 2^-115 is smaller than every u
-// (>= 2^-33), and both reject.  Only the band case branches.
-__device__ __forceinline__ bool accept_fast(double d, uint32_t x)
+// Same decision as accept_exact, faster on the device, in the log domain:
+// u < exp(d) iff ln u < d.  ln u is estimated in FP32 from the word alone
+// (hardware log2 of the FP32 u, times ln 2), so it can be computed before d
+// is known; d is compared in FP32 against ln u +- 2^-14.  Error of the
+// estimate for u in [2^-33, 1): FP32 u within 2^-23 relative (ln error <
+// 2^-22), log2 within 2 ulp of a result of magnitude < 33 (< 2^-17), the
+// ln 2 product 1 ulp (< 2^-19); FP32 rounding of d near the boundary
+// (|d| < 23) < 2^-19.  The sum is < 2^-15.5, so outside the band both
+// comparisons agree; d >= 0 accepts (u < 1); only the band case, d within
+// 2^-14 of ln u, falls back to the FP64 rule and branches.
+struct accept_band {
+  float lo, hi;
+};
+
+__device__ __forceinline__ accept_band accept_bounds(uint32_t x)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const float e = __builtin_amdgcn_exp2f(static_cast<float>(d * 0x1.71547652b82fep+0));
-#else
-  const float e = 0.0f;  // the host pass never executes device code
-#endif
   const float u = __builtin_fmaf(static_cast<float>(x), 0x1p-32f, 0x1p-33f);
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float lu = __builtin_amdgcn_logf(u) * 0x1.62e430p-1f;
+#else
+  const float lu = 0.0f * u;  // the host pass never executes device code
+#endif
+  return accept_band{lu - 0x1p-14f, lu + 0x1p-14f};
+}
+
+__device__ __forceinline__ bool accept_banded(double d, uint32_t x, accept_band band)
+{
+  const float df = static_cast<float>(d);
   const bool pos = d >= 0.0;
-  const bool tiny = d < -80.0;
-  const bool fa = u < e * (1.0f - 0x1p-12f);
-  const bool fr = u > e * (1.0f + 0x1p-12f);
-  bool acc = pos | (!tiny & fa);
-  const bool amb = !pos & !tiny & !fa & !fr;
-  if (amb) {
+  const bool fa = df > band.hi;
+  const bool fr = df < band.lo;
+  bool acc = pos | fa;
+  if (!pos & !fa & !fr) {
     acc = u32_to_unit(x) < wl_exp(d);
   }
   return acc;
+}
+
+__device__ __forceinline__ bool accept_fast(double d, uint32_t x)
+{
+  return accept_banded(d, x, accept_bounds(x));
 }
 #endif
 

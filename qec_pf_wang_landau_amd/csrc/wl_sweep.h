@@ -148,7 +148,8 @@ __device__ __forceinline__ attempt_reads attempt_issue(const sweep_ctx& c, const
 // every effect is a select or an LDS atomic whose operand is zero when the
 // effect does not apply.
 template<int BOUNDARY>
-__device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& s, const attempt_reads& r, uint32_t xa)
+__device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& s, const attempt_reads& r, uint32_t xa,
+                                               accept_band band)
 {
   const int j = r.j;
   const uint32_t sbit = (r.rc >> j) & 1u;
@@ -185,7 +186,7 @@ __device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& 
   g_new = (de == r.u2) ? g2 : g_new;
   g_new = (de == r.u1) ? g1 : g_new;
   const bool valid = s.alive & flag;
-  const bool acc = valid & in_range & accept_fast(s.g - g_new, xa);
+  const bool acc = valid & in_range & accept_banded(s.g - g_new, xa, band);
   atomicXor(&c.L[r.i], acc ? (1u << j) : 0u);
   s.e = acc ? en : s.e;
   s.b = acc ? s.b + (de >> c.shift) : s.b;
@@ -213,7 +214,7 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
   philox4 cur = rng.eval(blk);
   if ((att & 1u) && it < n) {  // odd start: second half of the block
     const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
-    attempt_finish<BOUNDARY>(cx, ws, r, cur.v[3]);
+    attempt_finish<BOUNDARY>(cx, ws, r, cur.v[3], accept_bounds(cur.v[3]));
     ++att;
     ++it;
     ++blk;
@@ -225,24 +226,27 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
     // of the next block between an attempt's reads and their first use.
     philox_run nx;
     nx.begin(blk + 1, rng.slot, rng.key0);
+    // acceptance bands depend on the words only: off the attempts' chains
+    const accept_band b1 = accept_bounds(cur.v[1]);
+    const accept_band b2 = accept_bounds(cur.v[3]);
     const attempt_reads r1 = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
     nx.fence();
     nx.rounds5();
     nx.fence();
-    attempt_finish<BOUNDARY>(cx, ws, r1, cur.v[1]);
+    attempt_finish<BOUNDARY>(cx, ws, r1, cur.v[1], b1);
     const bool first_ok = ws.alive;
     const attempt_reads r2 = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
     nx.fence();
     nx.rounds5();
     nx.fence();
-    attempt_finish<BOUNDARY>(cx, ws, r2, cur.v[3]);  // no effect once dead
+    attempt_finish<BOUNDARY>(cx, ws, r2, cur.v[3], b2);  // no effect once dead
     att += first_ok ? 2 : 1;  // the attempt that met the new energy counts, later ones do not
     ++blk;
     cur = philox4{{nx.c0, nx.c1, nx.c2, nx.c3}};
   }
   if (it < n && ws.alive) {
     const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
-    attempt_finish<BOUNDARY>(cx, ws, r, cur.v[1]);
+    attempt_finish<BOUNDARY>(cx, ws, r, cur.v[1], accept_bounds(cur.v[1]));
     ++att;
   }
 }

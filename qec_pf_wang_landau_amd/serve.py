@@ -8,12 +8,15 @@ Wang-Landau run of each class to ln f <= 1e-6 (src/mainrun.cu:243-383), the
 stitched ln g (src/result_handling.cpp:368-431), ln Z at the Nishimori
 temperature and the most likely class.
 
-Scheduling is iteration-level: the device runs slices of R rounds over every
-admitted task (one wl_fused_launch on a side stream); between slices the host
-admits waiting requests whose state fits the free HBM arenas, collects
-finished tasks and frees their memory.  A new request's couplings and prerun
-run on the default stream while a slice runs, so set-up overlaps the
-Wang-Landau work of the requests already on the device.  The kernel resumes
+Scheduling is iteration-level: per lattice size, the device runs slices of R
+rounds over every admitted task of that size (one wl_fused_launch on that
+size's stream); the sizes' slices run side by side and each one is collected
+and relaunched as soon as it ends, so small requests are not held back by
+large ones.  Between slices the host admits waiting requests whose state fits
+the free HBM arenas, collects finished tasks and frees their memory.  A new
+request's couplings and prerun (its four classes in one launch) run on the
+default stream while slices run, so set-up overlaps the Wang-Landau work of
+the requests already on the device.  The kernel resumes
 each task exactly where the last slice stopped, so a request's result does
 not depend on which other requests shared its slices: it is bit-identical
 to decoding it alone.
@@ -126,14 +129,14 @@ class Server:
     """HBM-resident decoding service (one GPU)."""
 
     def __init__(self, hbm_bytes: int = 1 << 30, slice_rounds: int = 200, max_tasks: int = 2048,
-                 prerun_walkers: int = 128, prerun_iters: int = 20000, exchange_every: int = 20,
+                 prerun_walkers: int = 128, prerun_iters_per_site: int = 2000, exchange_every: int = 20,
                  num_iterations: int = 1000, alpha: float = 0.8, beta: float = 1e-6, sizes: dict | None = None):
         torch = engine._torch()
         self.torch = torch
         self.sizes = dict(SIZES if sizes is None else sizes)
         self.slice_rounds = slice_rounds
         self.max_tasks = max_tasks
-        self.prerun_walkers, self.prerun_iters = prerun_walkers, prerun_iters
+        self.prerun_walkers, self.prerun_iters_per_site = prerun_walkers, prerun_iters_per_site
         self.run = dict(exchange_every=exchange_every, num_iterations=num_iterations, alpha=alpha, beta=beta)
         # Split the HBM budget over the arenas in proportion to the largest request's needs.
         need = self._footprint(max(self.sizes))
@@ -152,6 +155,7 @@ class Server:
         self.streams = {L: torch.cuda.Stream() for L in self.sizes}
         self.queues = {L: torch.zeros(1, dtype=torch.int32, device="cuda") for L in self.sizes}
         self.live: list[_Live] = []
+        self.inflight: dict = {}  # L -> (owners, d_tasks, d_small, done event) of the running slice
         self.waiting: list[Request] = []
         self.results: list[Result] = []
         self.slices = 0
@@ -211,16 +215,21 @@ class Server:
             self.live.append(_Live(req, st, offs, sizes, tasks, small, time.perf_counter()))
 
     def _setup(self, req: Request) -> engine.MainrunState:
-        """Couplings and prerun on the GPU, then the mainrun state of the four
+        """Couplings of the four classes (src/prerun.cu:52-75), their preruns
+        as one launch of four samples, then the mainrun state of the four
         class tasks (src/mainrun.cu:42-228); every class keys its walkers
         with the request's seed."""
         cfg = self.config(req.L)
+        L, N = req.L, req.L * req.L
+        J = np.concatenate([engine.disorder_gpu(req.seed, 1, L, L, req.p, cls) for cls in engine.ERROR_CLASSES])
+        pre = engine.prerun_gpu(J, L, L, cfg.boundary, req.seed, 4, self.prerun_walkers,
+                                self.prerun_iters_per_site * N, max(16, 2 * cfg.num_intervals))
         samples = []
-        for cls in engine.ERROR_CLASSES:
-            smp = engine.problem_gpu(cfg, 1, req.p, cls, prerun_walkers=self.prerun_walkers,
-                                     prerun_iters=self.prerun_iters, prerun_intervals=max(16, 2 * cfg.num_intervals),
-                                     sample_base=req.seed)[0]
-            samples.append(smp)
+        for c in range(4):
+            h, fl, fe = pre[c]
+            emin, emax, mask = engine.energy_spectrum(h, L, L)
+            samples.append(engine.SampleInput(J=J[c * 2 * N:(c + 1) * 2 * N].copy(), e_min=emin, e_max=emax,
+                                              mask=mask, found=fl, found_energy=fe, rng_key=req.seed))
         return engine.build_state(samples, cfg)
 
     def _upload(self, st: engine.MainrunState, offs: dict) -> None:
@@ -245,7 +254,7 @@ class Server:
         """Compact task table of the unfinished tasks of size L and their
         small arrays."""
         rows, small, owners = [], {k: [] for k in _PER_TASK}, []
-        for li, lv in enumerate(self.live):
+        for lv in self.live:
             if lv.req.L != L:
                 continue
             for q in range(4):
@@ -253,7 +262,7 @@ class Server:
                     rows.append(lv.tasks[q:q + 1])
                     for k in _PER_TASK:
                         small[k].append(lv.small[k][q:q + 1])
-                    owners.append((li, q))
+                    owners.append((lv, q))
         if not rows:
             return None, None, owners
         return np.concatenate(rows), {k: np.concatenate(v) for k, v in small.items()}, owners
@@ -273,20 +282,21 @@ class Server:
              engine.dptr(d_small["cond_sample"]), d["start"], d["end"], d["spectrum"],
              engine.dptr(d_small["counter"]), engine.dptr(d_small["rounds"]), engine.dptr(d_small["status"]),
              engine.dptr(self.queues[L]), None, self.slice_rounds, C.c_void_p(stream.cuda_stream))
-        return d_tasks, d_small
+        done = torch.cuda.Event()
+        done.record(stream)
+        return d_tasks, d_small, done
 
-    def _collect(self, launched: list) -> None:
-        for L, owners, d_tasks, d_small in launched:
-            self.streams[L].synchronize()
-            small = {k: engine.to_host(v, np.int64 if k in ("counter", "rounds") else np.int32)
-                     for k, v in d_small.items()}
-            for j, (li, q) in enumerate(owners):
-                lv = self.live[li]
-                for k in _PER_TASK:
-                    lv.small[k][q] = small[k][j]
-                if small["status"][j] != 0:
-                    lv.done_tasks[q] = True
-            del d_tasks
+    def _collect(self, L: int) -> None:
+        owners, d_tasks, d_small, _ = self.inflight.pop(L)
+        self.streams[L].synchronize()
+        small = {k: engine.to_host(v, np.int64 if k in ("counter", "rounds") else np.int32)
+                 for k, v in d_small.items()}
+        for j, (lv, q) in enumerate(owners):
+            for k in _PER_TASK:
+                lv.small[k][q] = small[k][j]
+            if small["status"][j] != 0:
+                lv.done_tasks[q] = True
+        del d_tasks
         finished = [lv for lv in self.live if lv.done_tasks.all()]
         if finished:
             self._decode(finished)
@@ -337,21 +347,24 @@ class Server:
 
     # ----------------------------------------------------------------- loop --
     def step(self, now: float) -> bool:
-        """One slice: launch over the admitted tasks, admit arrivals while it
-        runs, collect.  Returns False when there was nothing to run."""
-        launched = []
+        """Launch a slice for every size that has work and no slice running,
+        admit arrivals while slices run, collect the slices that ended.
+        Returns False when nothing ended or was launched."""
+        progressed = False
         for L in self.sizes:
+            if L in self.inflight:
+                continue
             tasks, small, owners = self._table(L)
             if tasks is not None:
-                d_tasks, d_small = self._launch(L, tasks, small)
-                launched.append((L, owners, d_tasks, d_small))
-        if not launched:
-            self._admit(now)
-            return bool(self.live)
+                self.inflight[L] = (owners, *self._launch(L, tasks, small))
+                self.slices += 1
+                progressed = True
         self._admit(now)  # overlaps the slices
-        self._collect(launched)
-        self.slices += 1
-        return True
+        for L in list(self.inflight):
+            if self.inflight[L][3].query():
+                self._collect(L)
+                progressed = True
+        return progressed
 
     def serve(self, requests: list[Request], t0: float | None = None, drain_s: float = 600.0) -> list[Result]:
         """Requests arrive at t0 + arrival (seconds); runs until every one is
@@ -362,13 +375,15 @@ class Server:
         for r in pending:
             r.arrival = t0 + r.arrival
             self.submit(r)
-        while self.waiting or self.live:
+        while self.waiting or self.live or self.inflight:
             now = time.perf_counter()
             if not self.step(now):
-                nxt = self.waiting[0].arrival if self.waiting else now
+                nxt = self.waiting[0].arrival if (self.waiting and not self.live) else now + 5e-4
                 time.sleep(max(0.0, min(nxt - now, 0.01)))
             if now - t0 > last + drain_s:
                 break
+        for L in list(self.inflight):
+            self._collect(L)
         return self.results
 
 

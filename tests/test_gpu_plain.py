@@ -68,7 +68,7 @@ def test_math_host_device_bit_exact(cuda):
 
 
 def test_fast_acceptance_equals_exact_rule(cuda):
-    """The sweep's FP32 estimate + exact fallback decides exactly as
+    """The sweep's FP32 log-domain estimate + exact fallback decides exactly as
     u < min(1, exp(d)) computed in FP64 by the oracle, including words
     placed within a few ulps of the boundary."""
     rng = np.random.default_rng(11)
@@ -79,6 +79,9 @@ def test_fast_acceptance_equals_exact_rule(cuda):
         e = math.exp(d[i]) if d[i] < 0 else 1.0
         xb = int(e * 2**32 - 0.5)
         x[i] = np.uint32(min(max(xb + (i % 5) - 2, 0), 2**32 - 1))
+    # d within a few 2^-14 of ln u: both edges of the FP32 band and its inside
+    lu = np.log((x[200000:300000].astype(np.float64) + 0.5) * 2.0**-32)
+    d[200000:300000] = lu + rng.uniform(-4e-4, 4e-4, lu.size)
     o = oracle()
     expect = np.array([1 if (float(xi) + 0.5) / 2**32 < (1.0 if di >= 0 else o.wlo_exp(float(di))) else 0
                        for di, xi in zip(d, x)], np.int8)
