@@ -348,11 +348,14 @@ def uneven_states(engine, n_tasks: int, seed: int = 2024):
             pre = engine.prerun_gpu(J, L, L, int(b), seed + 7919 * L + b, ids.size, 4 * I, 4000 * L * L, 4 * I)
             N = L * L
             samples = []
+            # walker keys consecutive within a shape group, as the plain loop
+            # keys the samples of one run (seed + sample, src/wl.cu:1074)
+            key0 = seed + 100003 * len(states)
             for j, q in enumerate(ids):
                 h, fl, fe = pre[j]
                 emin, emax, mask = engine.energy_spectrum(h, L, L)
                 samples.append(engine.SampleInput(J=J[j * 2 * N:(j + 1) * 2 * N].copy(), e_min=emin, e_max=emax,
-                                                  mask=mask, found=fl, found_energy=fe, rng_key=seed + int(q)))
+                                                  mask=mask, found=fl, found_energy=fe, rng_key=key0 + j))
             states.append(engine.build_state(samples, cfg))
             log(f"uneven: L={L} boundary={b}: {ids.size} tasks ready")
     return states
@@ -375,9 +378,11 @@ def bench_uneven(args, engine, torch) -> dict:
     attempts = sum(int(pt.rounds.sum()) * pt.cfg.num_intervals * pt.cfg.walker_per_interval * pt.cfg.num_iterations
                    for pt in parts)
     finished = sum(int((pt.status == 1).sum()) for pt in parts)
-    log(f"uneven: {R} rounds in {dt:.2f}s, {attempts:.3e} attempts, {finished} tasks finished")
+    status = np.bincount(np.concatenate([pt.status for pt in parts]), minlength=6).tolist()
+    log(f"uneven: {R} rounds in {dt:.2f}s, {attempts:.3e} attempts, task status counts {status}")
     plain_s = None
     identical = None
+    compared = 0
     if not args.no_plain:
         plain_s = 0.0
         identical = True
@@ -385,12 +390,17 @@ def bench_uneven(args, engine, torch) -> dict:
             ds = engine.DeviceState(st.copy())
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            engine.run_plain(ds, R)
+            rc, _ = engine.run_plain(ds, R)
             torch.cuda.synchronize()
             plain_s += time.perf_counter() - t1
-            got = ds.pull()
-            identical &= bool(np.array_equal(got.logG, pt.logG) and np.array_equal(got.lattice, pt.lattice))
-        log(f"uneven plain: {plain_s:.2f}s, identical {identical}")
+            # A walker meeting an energy the prerun missed stops the whole
+            # plain run (the reference aborts the mainrun) but only its own
+            # task in the batched forms: compare the groups without one.
+            if rc == 0 and (pt.status < 2).all():
+                got = ds.pull()
+                identical &= bool(np.array_equal(got.logG, pt.logG) and np.array_equal(got.lattice, pt.lattice))
+                compared += 1
+        log(f"uneven plain: {plain_s:.2f}s, identical {identical} over {compared} of {len(states)} shape groups")
     achieved = attempts * BYTES_PER_ATTEMPT / dt / 1e9
     return {
         "metric": METRIC, "value": attempts / dt, "unit": "attempts/s", "n_gpus": 1, "steps": 1, "warmup": 0,
@@ -402,9 +412,9 @@ def bench_uneven(args, engine, torch) -> dict:
                                "autotuner, both paths concurrently",
                    "tasks": args.tasks, "rounds": R},
         "paths": {f"L{st.cfg.X}_b{st.cfg.boundary}": forms[i][0] for i, st in enumerate(states)},
-        "tasks_finished": finished,
+        "tasks_finished": finished, "task_status_counts": status,
         "plain_seconds": plain_s, "batched_vs_plain": (plain_s / dt) if plain_s else None,
-        "plain_bit_identical": identical,
+        "plain_bit_identical": identical, "plain_groups_compared": compared,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": None,

@@ -448,6 +448,8 @@ def run_plain(ds: DeviceState, max_rounds: int = 1 << 40) -> tuple[int, int]:
     loop ended by convergence when rounds < max_rounds."""
     st = ds.host
     cfg = st.cfg
+    if not np.array_equal(st.keys, st.keys[0] + np.arange(st.S, dtype=st.keys.dtype)):
+        raise ValueError("the plain loop keys sample s with seed + s: the state's keys must be consecutive")
     counter = C.c_int64(ds.counter)
     rounds = C.c_int64(0)
     rc = lib().wl_mainrun_loop(
@@ -474,6 +476,12 @@ def run_fused(ds: DeviceState, tasks: np.ndarray, max_rounds: int = 1 << 40, d_t
 
 # ------------------------------------------------------- GPU problem setup --
 
+def _cur_stream() -> C.c_void_p:
+    """The calling thread's current torch stream, so that library kernels and
+    torch's own copies and fills of the same tensors stay in one order."""
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
 class Keep:
     """Holds device tensors for the duration of queued kernels."""
 
@@ -491,11 +499,11 @@ def disorder_gpu(seed: int, S: int, X: int, Y: int, prob: float, cls: str) -> np
     torch = _torch()
     N = X * Y
     J = torch.zeros(S * 2 * N, dtype=torch.int8, device="cuda")
-    call("wl_init_interactions", dptr(J), N, S, seed, C.c_float(prob), None)
+    call("wl_init_interactions", dptr(J), N, S, seed, C.c_float(prob), _cur_stream())
     if cls in ("h", "c"):
-        call("wl_apply_x_horizontal_error", dptr(J), X, Y, S, None)
+        call("wl_apply_x_horizontal_error", dptr(J), X, Y, S, _cur_stream())
     if cls in ("v", "c"):
-        call("wl_apply_x_vertical_error", dptr(J), X, Y, S, None)
+        call("wl_apply_x_vertical_error", dptr(J), X, Y, S, _cur_stream())
     return to_host(J, np.int8)
 
 
@@ -513,15 +521,16 @@ def prerun_gpu(J: np.ndarray, X: int, Y: int, boundary: int, seed: int, S: int, 
     off = torch.zeros(total, dtype=torch.int32, device="cuda")
     E = torch.zeros(total, dtype=torch.int32, device="cuda")
     tJ = keep(J)
-    call("wl_init_lattice", dptr(lat), dptr(probs), X, Y, total, seed, walkers, None)
-    call("wl_init_offsets_lattice", dptr(off), X, Y, total, None)
-    call("wl_calc_energy", boundary, dptr(E), dptr(lat), dptr(tJ), dptr(off), X, Y, total, walkers, None)
+    call("wl_init_lattice", dptr(lat), dptr(probs), X, Y, total, seed, walkers, _cur_stream())
+    call("wl_init_offsets_lattice", dptr(off), X, Y, total, _cur_stream())
+    call("wl_calc_energy", boundary, dptr(E), dptr(lat), dptr(tJ), dptr(off), X, Y, total, walkers, _cur_stream())
     hist = torch.zeros(S * (4 * N + 1), dtype=torch.int64, device="cuda")
     so = torch.zeros(total, dtype=torch.int64, device="cuda")
     flag = torch.zeros(S * num_intervals, dtype=torch.int32, device="cuda")
     found = torch.zeros(S * num_intervals * N, dtype=torch.int8, device="cuda")
     call("wl_wang_landau_pre_run", dptr(E), dptr(flag), dptr(lat), dptr(found), dptr(hist), dptr(so), dptr(tJ),
-         dptr(off), -2 * N, 2 * N, iters, X, Y, seed, num_intervals, iv.len_interval, total, walkers, boundary, None)
+         dptr(off), -2 * N, 2 * N, iters, X, Y, seed, num_intervals, iv.len_interval, total, walkers, boundary,
+         _cur_stream())
     h_hist = to_host(hist, np.uint64)
     h_found = to_host(found, np.int8)
     out = []

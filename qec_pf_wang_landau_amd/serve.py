@@ -12,11 +12,13 @@ Scheduling is iteration-level: per lattice size, the device runs slices of R
 rounds over every admitted task of that size (one wl_fused_launch on that
 size's stream); the sizes' slices run side by side and each one is collected
 and relaunched as soon as it ends, so small requests are not held back by
-large ones.  Between slices the host admits waiting requests whose state fits
-the free HBM arenas, collects finished tasks and frees their memory.  A new
-request's couplings and prerun (its four classes in one launch) run on the
-default stream while slices run, so set-up overlaps the Wang-Landau work of
-the requests already on the device.  The kernel resumes
+large ones.  Between slices the host reserves HBM arena space for arrived
+requests (FIFO; a request waits while its footprint does not fit), collects
+finished tasks and frees their memory.  Each admitted request's couplings and
+prerun (its four classes in one launch) run in a pool of set-up threads, each
+on its own stream, so set-up of many requests overlaps itself and the
+Wang-Landau work already on the device; a request boards the slices once its
+set-up is done.  The kernel resumes
 each task exactly where the last slice stopped, so a request's result does
 not depend on which other requests shared its slices: it is bit-identical
 to decoding it alone.
@@ -24,8 +26,10 @@ to decoding it alone.
 
 from __future__ import annotations
 
+import concurrent.futures as cf
 import ctypes as C
 import math
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -130,9 +134,11 @@ class Server:
 
     def __init__(self, hbm_bytes: int = 1 << 30, slice_rounds: int = 200, max_tasks: int = 2048,
                  prerun_walkers: int = 128, prerun_iters_per_site: int = 2000, exchange_every: int = 20,
-                 num_iterations: int = 1000, alpha: float = 0.8, beta: float = 1e-6, sizes: dict | None = None):
+                 num_iterations: int = 1000, alpha: float = 0.8, beta: float = 1e-6, sizes: dict | None = None,
+                 setup_workers: int = 8):
         torch = engine._torch()
         self.torch = torch
+        engine.lib()  # load the library once, before any set-up thread calls it
         self.sizes = dict(SIZES if sizes is None else sizes)
         self.slice_rounds = slice_rounds
         self.max_tasks = max_tasks
@@ -156,6 +162,10 @@ class Server:
         self.queues = {L: torch.zeros(1, dtype=torch.int32, device="cuda") for L in self.sizes}
         self.live: list[_Live] = []
         self.inflight: dict = {}  # L -> (owners, d_tasks, d_small, done event) of the running slice
+        self.preparing: list = []  # (request, reserved offsets, reserved sizes, admitted time, set-up future)
+        self.setup_workers = setup_workers
+        self.pool = None
+        self._tls = threading.local()
         self.waiting: list[Request] = []
         self.results: list[Result] = []
         self.slices = 0
@@ -182,26 +192,44 @@ class Server:
     def submit(self, req: Request) -> None:
         self.waiting.append(req)
 
+    def _reserve(self, need: dict) -> dict | None:
+        offs = {}
+        for k, n in need.items():
+            o = self.arena[k].alloc(n)
+            if o is None:
+                for kk, oo in offs.items():
+                    self.arena[kk].release(oo, need[kk])
+                return None
+            offs[k] = o
+        return offs
+
     def _admit(self, now: float) -> None:
-        """FIFO admission of arrived requests while the arenas have room."""
-        while self.waiting and self.waiting[0].arrival <= now and 4 * (len(self.live) + 1) <= self.max_tasks:
-            if self.waiting[0].L not in self.sizes:
-                raise ValueError(f"request {self.waiting[0].rid}: lattice size {self.waiting[0].L} not served")
+        """Reserve memory for arrived requests (FIFO; the head waits while its
+        footprint does not fit) and start their set-up; board the requests
+        whose set-up has finished."""
+        while (self.waiting and self.waiting[0].arrival <= now
+               and 4 * (len(self.live) + len(self.preparing) + 1) <= self.max_tasks):
             req = self.waiting[0]
-            if not self._fits(self._footprint(req.L)):
-                return  # head of line waits for memory
-            st = self._setup(req)
+            if req.L not in self.sizes:
+                raise ValueError(f"request {req.rid}: lattice size {req.L} not served")
+            need = self._footprint(req.L)
+            offs = self._reserve(need)
+            if offs is None:
+                break  # head of line waits for memory
+            self.waiting.pop(0)
+            fut = self.pool.submit(self._setup_on_worker, req)
+            self.preparing.append((req, offs, need, time.perf_counter(), fut))
+        still = []
+        for item in self.preparing:
+            req, offs, need, admitted, fut = item
+            if not fut.done():
+                still.append(item)
+                continue
+            st = fut.result()
             sizes = {"lattice": st.lattice.size, "J": st.J.size, "hist": st.logG.size, "walker": st.E.size,
                      "interval": st.start.size, "spectrum": st.spectrum.size}
-            offs = {}
-            for k, n in sizes.items():
-                o = self.arena[k].alloc(n)
-                if o is None:
-                    for kk, oo in offs.items():
-                        self.arena[kk].release(oo, sizes[kk])
-                    return  # head of line waits for memory
-                offs[k] = o
-            self.waiting.pop(0)
+            if any(sizes[k] > need[k] for k in sizes):
+                raise RuntimeError(f"request {req.rid}: state exceeds its reserved footprint")
             self._upload(st, offs)
             tasks = engine.make_tasks(st)
             tasks["walker_off"] += offs["walker"]
@@ -212,7 +240,18 @@ class Server:
             tasks["spectrum_off"] += offs["spectrum"]
             tasks["couplings_off"] += offs["J"]
             small = {k: getattr(st, k).copy() for k in _PER_TASK}
-            self.live.append(_Live(req, st, offs, sizes, tasks, small, time.perf_counter()))
+            self.live.append(_Live(req, st, offs, dict(need), tasks, small, admitted))
+        self.preparing = still
+
+    def _setup_on_worker(self, req: Request) -> engine.MainrunState:
+        """_setup on the calling set-up thread's own stream."""
+        torch = self.torch
+        s = getattr(self._tls, "stream", None)
+        if s is None:
+            s = torch.cuda.Stream()
+            self._tls.stream = s
+        with torch.cuda.stream(s):
+            return self._setup(req)
 
     def _setup(self, req: Request) -> engine.MainrunState:
         """Couplings of the four classes (src/prerun.cu:52-75), their preruns
@@ -370,20 +409,26 @@ class Server:
         """Requests arrive at t0 + arrival (seconds); runs until every one is
         decoded or drain_s after the last arrival."""
         t0 = time.perf_counter() if t0 is None else t0
+        self.pool = cf.ThreadPoolExecutor(max_workers=self.setup_workers)
         pending = sorted(requests, key=lambda r: r.arrival)
         last = pending[-1].arrival if pending else 0.0
         for r in pending:
             r.arrival = t0 + r.arrival
             self.submit(r)
-        while self.waiting or self.live or self.inflight:
-            now = time.perf_counter()
-            if not self.step(now):
-                nxt = self.waiting[0].arrival if (self.waiting and not self.live) else now + 5e-4
-                time.sleep(max(0.0, min(nxt - now, 0.01)))
-            if now - t0 > last + drain_s:
-                break
-        for L in list(self.inflight):
-            self._collect(L)
+        try:
+            while self.waiting or self.live or self.inflight or self.preparing:
+                now = time.perf_counter()
+                if not self.step(now):
+                    busy = self.live or self.preparing
+                    nxt = self.waiting[0].arrival if (self.waiting and not busy) else now + 5e-4
+                    time.sleep(max(0.0, min(nxt - now, 0.01)))
+                if now - t0 > last + drain_s:
+                    break
+            for L in list(self.inflight):
+                self._collect(L)
+        finally:
+            self.pool.shutdown(wait=True)
+            self.pool = None
         return self.results
 
 

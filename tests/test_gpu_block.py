@@ -102,3 +102,23 @@ def test_block_matches_fused_on_6x6_batch(cuda):
     for k, v in fused.arrays.items():
         assert np.array_equal(v, blk.arrays[k]), k
     assert (fused.arrays["status"] == 1).all()
+
+
+def test_mixed_run_and_autotuner_bit_exact(cuda):
+    """MixedRun runs one-workgroup tasks and per-interval tasks side by side
+    on two streams; the autotuner's choice changes nothing in the results."""
+    states = [
+        make_state(4, 4, 0, 4, 4, 2, 0.1, "i", n=300, c=4),
+        make_state(5, 5, 1, 5, 6, 1, 0.15, "v", n=400, c=3),
+        make_state(6, 4, 1, 3, 40, 1, 0.2, "h", n=200, c=2),
+    ]
+    refs = [oracle_per_task(st, 50) for st in states]
+    forms = engine.autotune_paths(states, rounds=3)
+    assert set(forms) == {0, 1, 2}
+    assert all(f in ("fused", "block") for f, _ in forms.values())
+    for choice in (forms, {0: ("fused", {}), 1: ("block", {}), 2: ("fused", {})},
+                   {0: ("block", {}), 1: ("fused", {}), 2: ("block", {})}):
+        run = engine.MixedRun(states, choice)
+        run.run(50)
+        for got, rf in zip(run.pull(), refs):
+            compare_tasks(got, rf)

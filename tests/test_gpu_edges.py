@@ -1,0 +1,94 @@
+"""Edge cases of the batched forms: empty task tables, the widest lattice
+rows (Y = 32 fills every bit of a row word), a large interval (600 walkers,
+the per-interval path's 1024-thread variant), and inputs the
+kernels must refuse instead of indexing outside their LDS (walker energy
+off the interval, histogram count beyond 32 bits)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import compare_tasks, make_state, oracle_per_task, oracle_samples
+from qec_pf_wang_landau_amd import engine
+from qec_pf_wang_landau_amd._lib import call
+
+pytestmark = pytest.mark.gpu
+
+
+def test_empty_task_tables(cuda):
+    st = make_state(4, 4, 0, 2, 4, 1, 0.1, "v")
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    empty = np.zeros(0, engine.TASK_DTYPE)
+    call("wl_fused_run", engine.dptr(dev.d_tasks), engine._p(np.zeros(1, engine.TASK_DTYPE)), 0, dev["lattice"],
+         dev["J"], dev["H"], dev["logG"], dev["factor"], dev["E"], dev["off_iter"], dev["off_lattice"], dev["idx"],
+         dev["cond"], dev["cond_sample"], dev["start"], dev["end"], dev["spectrum"], dev["counter"], dev["rounds"],
+         dev["status"], 10, None)
+    blocks = engine.C.c_int64(-1)
+    call("wl_block_run", engine.dptr(dev.d_tasks), engine._p(np.zeros(1, engine.TASK_DTYPE)), 0, dev["lattice"],
+         dev["J"], dev["H"], dev["logG"], dev["factor"], dev["E"], dev["off_iter"], dev["off_lattice"], dev["idx"],
+         dev["cond"], dev["cond_sample"], dev["start"], dev["end"], dev["spectrum"], dev["counter"], dev["rounds"],
+         dev["status"], 10, 64, 1, engine.C.byref(blocks), None)
+    assert blocks.value == 0 and empty.size == 0
+    got = dev.pull()
+    for k, v in engine.merge_states([st]).arrays.items():
+        assert np.array_equal(got.arrays[k], v), k
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_widest_rows_both_forms(cuda, boundary):
+    """Y = 32: a lattice row is one full u32 word in LDS."""
+    cfg = engine.RunConfig(X=4, Y=32, boundary=boundary, num_intervals=3, walker_per_interval=4, overlap=0.25,
+                           alpha=0.8, beta=1e-6, num_iterations=200, exchange_every=3, seed=12345)
+    st = engine.build_state(oracle_samples(cfg, 1, 0.1, "h", prerun_walkers=64, prerun_iters=20000), cfg)
+    refs = oracle_per_task(st, 12)
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    dev.run(12)
+    compare_tasks(dev.pull().split()[0], refs)
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    dev.run_blocks(12)
+    compare_tasks(dev.pull().split()[0], refs)
+
+
+def test_largest_interval_block_path(cuda):
+    """600 walkers per interval (not a multiple of the wavefront): the
+    1024-thread variant of the per-interval kernel, two intervals, exchange
+    every round.  1024 walkers of this lattice would need 164 KB of LDS per
+    interval, which wl_block_run refuses."""
+    st = make_state(4, 4, 0, 2, 600, 1, 0.1, "i", n=30, c=1)
+    refs = oracle_per_task(st, 4)
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    dev.run_blocks(4)
+    compare_tasks(dev.pull().split()[0], refs)
+
+
+def _corrupt(st, what):
+    bad = st.copy()
+    if what == "energy":
+        bad.E[3] = bad.end[0] + 4  # walker 3 of interval 0 placed outside its interval
+    else:
+        bad.H[int(bad.off_hist[0])] = np.uint64(1) << np.uint64(33)
+    return bad
+
+
+@pytest.mark.parametrize("what,code", [("energy", 5), ("hist", 4)])
+def test_refused_inputs(cuda, what, code):
+    good = make_state(4, 4, 0, 3, 4, 1, 0.1, "v", n=100, c=2)
+    bad = _corrupt(good, what)
+    for runner in ("fused", "block"):
+        dev = engine.BatchDevice(engine.merge_states([bad, good]))
+        if runner == "fused":
+            dev.run(5)
+        else:
+            dev.run_blocks(5)
+        got = dev.pull()
+        assert got.arrays["status"][0] == code, (runner, got.arrays["status"])
+        # the healthy task of the same call is unaffected
+        compare_tasks(got.split()[1], oracle_per_task(good, 5))
+
+
+def test_oversized_interval_is_refused(cuda):
+    st = make_state(4, 4, 0, 2, 1024, 1, 0.1, "i", n=30, c=1)
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    with pytest.raises(Exception, match="does not fit in LDS"):
+        dev.run_blocks(4)
