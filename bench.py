@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--duration", type=float, default=10.0, help="serve: arrival window (s)")
     ap.add_argument("--hbm-gib", type=float, default=8.0, help="serve: HBM budget of the walker state (GiB)")
     ap.add_argument("--tasks", type=int, default=1000, help="uneven: number of tasks")
+    ap.add_argument("--exact", action="store_true",
+                    help="6x6: after timing, enumerate the exact DOS of every task on the GPU and report the "
+                         "decoded ln g error and the agreement of the class decisions")
     ap.add_argument("--workload", choices=("6x6", "16x16", "32x32", "serve", "uneven"), default="6x6",
                     help="6x6: the 400-task batch (default); 16x16: periodic p=0.1 samples, 64 intervals x 64 "
                          "walkers, per-interval batched path vs the plain loop, both to ln f=1e-6; 32x32: one "
@@ -421,6 +424,39 @@ def bench_uneven(args, engine, torch) -> dict:
     }
 
 
+def exact_check(engine, torch, dev, batch, samples: int) -> dict:
+    """Gray-code exact DOS of every task (GPU) against the decoded ln g of the
+    timed run: max/mean abs error, and the class decision at the Nishimori
+    temperature of p = 0.04 against the exact decision."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    counts = engine.exact_counts_gpu(batch.arrays["J"], 6, 6, 0)
+    torch.cuda.synchronize()
+    t_exact = time.perf_counter() - t0
+    bN = engine.nishimori_beta(0.04)
+    dec = engine.stitch_tasks_gpu(dev, np.full(batch.tasks.size, bN))
+    with np.errstate(divide="ignore"):
+        ex = np.log(counts.astype(np.float64))
+    worst, means, lz_exact = 0.0, [], np.zeros(batch.tasks.size)
+    for q in range(batch.tasks.size):
+        err = np.abs(dec.lng[q] - ex[q][dec.E[q] + 72])
+        worst = max(worst, float(err.max()))
+        means.append(float(err.mean()))
+        fin = np.isfinite(ex[q])
+        lz_exact[q] = engine.log_partition((np.nonzero(fin)[0] - 72).astype(np.int32), ex[q][fin], bN)
+    lz = dec.log_z.reshape(4, samples).T
+    lze = lz_exact.reshape(4, samples).T
+    agree = engine.decide(lz) == engine.decide(lze)
+    srt = np.sort(lze, axis=1)
+    clear = (srt[:, -1] - srt[:, -2]) > 0.5
+    log(f"exact: {batch.tasks.size} tasks enumerated in {t_exact:.1f}s; ln g max {worst:.3f} mean "
+        f"{np.mean(means):.4f}; decisions agree {agree.mean():.3f} ({agree[clear].mean():.3f} where clear)")
+    return {"seconds": t_exact, "tasks": int(batch.tasks.size), "lng_max_abs": worst,
+            "lng_mean_abs": float(np.mean(means)), "decisions_agree": float(agree.mean()),
+            "decisions_agree_where_gap_gt_0.5": float(agree[clear].mean()) if clear.any() else None,
+            "samples_with_clear_gap": int(clear.sum())}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -477,6 +513,8 @@ def main():
     samples_per_s = args.samples / step_s
     print(f"[bench] fused: {step_s * 1e3:.1f} ms/step (kernel {kern_s * 1e3:.1f} ms), {attempts:.3e} attempts, "
           f"{n_ok}/{status.size} tasks finished", file=sys.stderr, flush=True)
+
+    exact = exact_check(engine, torch, dev, batch, args.samples) if args.exact else None
 
     plain = None
     if not args.no_plain:
@@ -538,6 +576,7 @@ def main():
             "traffic": measured_traffic(),
         },
         "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None,
+        "exact_check": exact,
     }
     print(json.dumps(out))
 

@@ -286,6 +286,16 @@ int wl_block_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, 
  * (host interval table of task->I entries); must be <= 160 KiB. */
 size_t wl_block_interval_lds_bytes(const wl_task* task, const int* h_start, const int* h_end);
 
+/* Exact density of states by Gray-code enumeration of all 2^N states
+ * (7 <= N = X Y <= 40; periodic lattices need X, Y >= 3): d_counts
+ * (num_samples x (4N+1), u64, overwritten) receives g(E) over [-2N, 2N] for
+ * each sample's couplings (d_J: num_samples x 2N, the layout of
+ * init_interactions, src/wl.cu:227-257).  The exact reference that the
+ * reference's mainrun approximates (its tests compare against enumeration,
+ * test/test_wang_landau_pre_run.cu:695-811); no counterpart kernel in the
+ * reference. */
+int wl_exact_dos(const int8_t* d_J, int X, int Y, int boundary, int num_samples, uint64_t* d_counts, void* stream);
+
 /* Workspace bytes wl_stitch_tasks needs for one task with I intervals whose
  * lengths sum to total_len. */
 size_t wl_stitch_work_bytes(int num_intervals, int total_len);

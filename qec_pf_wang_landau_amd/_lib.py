@@ -51,6 +51,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "wl_fused_launch": (I32, [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, P]),
     "wl_block_interval_lds_bytes": (SZ, [P, P, P]),
     "wl_block_run": (I32, [P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, I32, I32, P, P]),
+    "wl_exact_dos": (I32, [P, I32, I32, I32, I32, P, P]),
     "wl_stitch_work_bytes": (SZ, [I32, I32]),
     "wl_stitch_tasks": (I32, [P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P]),
     # host side (include/wl_host.h)

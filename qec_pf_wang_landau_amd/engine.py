@@ -746,3 +746,24 @@ class MixedRun:
                 for i, part in zip(idx, dev.pull().split()):
                     out[i] = part
         return [out[i] for i in sorted(out)]
+
+
+# ------------------------------------------------- exact density of states --
+
+def exact_counts_gpu(J: np.ndarray, X: int, Y: int, boundary: int) -> np.ndarray:
+    """wl_exact_dos: g(E) over [-2N, 2N] for each sample of J (S x 2N int8),
+    by Gray-code enumeration of all 2^N states; returns S x (4N+1) u64."""
+    torch = _torch()
+    N = X * Y
+    S = J.size // (2 * N)
+    tJ = to_device(np.ascontiguousarray(J, np.int8))
+    out = torch.zeros(S * (4 * N + 1), dtype=torch.int64, device="cuda")
+    call("wl_exact_dos", dptr(tJ), X, Y, boundary, S, dptr(out), _cur_stream())
+    return to_host(out, np.uint64).reshape(S, 4 * N + 1)
+
+
+def exact_lng_gpu(J: np.ndarray, X: int, Y: int, boundary: int) -> np.ndarray:
+    """ln g(E) over [-2N, 2N] from exact_counts_gpu (-inf where g = 0)."""
+    c = exact_counts_gpu(J, X, Y, boundary).astype(np.float64)
+    with np.errstate(divide="ignore"):
+        return np.log(c)
