@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
       const bool upd = misc[2] == 0;
       const double lnf = wl_log(*fptr);
       uint32_t* Lw = lat + tid * X;
-      sweep_state ws{EL[tid], 0, 0.0, true, 0};
+      sweep_state ws{EL[tid], 0, 0, 0.0, true, 0};
       ws.b = (ws.e - e0) >> shift;
       ws.g = G[ws.b * W + tid];
       const sweep_ctx cx{Lw, JJ, mbits, G + tid, Hc + tid, X, Y, emin, sk, ek, shift, W, upd, lnf};

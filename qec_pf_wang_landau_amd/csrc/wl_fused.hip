@@ -358,7 +358,7 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       const bool upd = Lcond[k] == 0;  // active interval: ln g and H are updated
       const double lnf = wl_log(Lf[k]);
       uint32_t* L = lat + SL[tid] * X;
-      sweep_state ws{EL[tid], 0, 0.0, true, 0};
+      sweep_state ws{EL[tid], 0, 0, 0.0, true, 0};
       ws.b = (ws.e - e0) >> shift;
       ws.g = Gw[ws.b * W];
       const sweep_ctx cx{L, JJ, mbits, Gw, Hw, X, Y, emin, sk, ek, shift, W, upd, lnf};

@@ -358,8 +358,8 @@ WL_HD bool accept_exact(double d, uint32_t x)
 // is known; d is compared in FP32 against ln u +- 2^-14.  Error of the
 // estimate for u in [2^-33, 1): FP32 u within 2^-23 relative (ln error <
 // 2^-22), log2 within 2 ulp of a result of magnitude < 33 (< 2^-17), the
-// ln 2 product 1 ulp (< 2^-19); FP32 rounding of d near the boundary
-// (|d| < 23) < 2^-19.  The sum is < 2^-15.5, so outside the band both
+// ln 2 product 1 ulp (< 2^-19); d itself is compared in FP64 against the
+// widened bounds.  The sum is < 2^-15.5, so outside the band both
 // comparisons agree; d >= 0 accepts (u < 1); only the band case, d within
 // 2^-14 of ln u, falls back to the FP64 rule and branches.
 struct accept_band {
@@ -379,10 +379,10 @@ __device__ __forceinline__ accept_band accept_bounds(uint32_t x)
 
 __device__ __forceinline__ bool accept_banded(double d, uint32_t x, accept_band band)
 {
-  const float df = static_cast<float>(d);
+  // the bounds widen to FP64 off the chain; d is compared without rounding
   const bool pos = d >= 0.0;
-  const bool fa = df > band.hi;
-  const bool fr = df < band.lo;
+  const bool fa = d > static_cast<double>(band.hi);
+  const bool fr = d < static_cast<double>(band.lo);
   bool acc = pos | fa;
   if (!pos & !fa & !fr) {
     acc = u32_to_unit(x) < wl_exp(d);

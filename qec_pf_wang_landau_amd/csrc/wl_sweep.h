@@ -39,7 +39,8 @@ constexpr int kGuard = 8;
 // Walker registers carried across attempts.
 struct sweep_state {
   int e;       // energy
-  int b;       // compact bin of e
+  int b;       // compact bin of e (set by the caller; the sweep tracks bw)
+  int bw;      // b * W: offset of the walker's current row in G and H
   double g;    // ln g at b (mirrors the LDS value)
   bool alive;  // no energy outside the spectrum seen yet
   int bad_e;   // the first energy outside the spectrum, once alive is false
@@ -135,11 +136,13 @@ __device__ __forceinline__ attempt_reads attempt_issue(const sweep_ctx& c, const
   const int rel = s.e - c.emin;  // >= 0: the current energy is a spectrum energy
   r.m0 = c.mbits[rel >> 5];
   r.m1 = c.mbits[(rel >> 5) + 1];
-  const double* Gb = c.G + __mul24(s.b, c.W);
-  r.g1 = Gb[__mul24(r.u1 >> c.shift, c.W)];
-  r.g2 = Gb[__mul24(r.u2 >> c.shift, c.W)];
-  r.g3 = Gb[-__mul24(r.u2 >> c.shift, c.W)];
-  r.g4 = Gb[-__mul24(r.u1 >> c.shift, c.W)];
+  const double* Gb = c.G + s.bw;
+  const int o1 = __mul24(r.u1 >> c.shift, c.W);
+  const int o2 = __mul24(r.u2 >> c.shift, c.W);
+  r.g1 = Gb[o1];
+  r.g2 = Gb[o2];
+  r.g3 = Gb[-o2];
+  r.g4 = Gb[-o1];
   return r;
 }
 
@@ -180,22 +183,23 @@ __device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& 
   // attempt's chain.
   double g1 = r.g1, g2 = r.g2, g3 = r.g3, g4 = r.g4;
   __asm__ volatile("" : "+v"(g1), "+v"(g2), "+v"(g3), "+v"(g4));
-  double g_new = s.g;
-  g_new = (de == -r.u1) ? g4 : g_new;
-  g_new = (de == -r.u2) ? g3 : g_new;
-  g_new = (de == r.u2) ? g2 : g_new;
-  g_new = (de == r.u1) ? g1 : g_new;
+  // two-level select: the compares run side by side, the chain is three
+  // selects deep
+  const double glo = (de == -r.u1) ? g4 : g3;  // de < 0
+  const double ghi = (de == r.u1) ? g1 : g2;   // de > 0
+  double g_new = (de < 0) ? glo : ghi;
+  g_new = (de == 0) ? s.g : g_new;
+  const int bw_new = s.bw + __mul24(de >> c.shift, c.W);  // off the decision's chain
   const bool valid = s.alive & flag;
   const bool acc = valid & in_range & accept_banded(s.g - g_new, xa, band);
   atomicXor(&c.L[r.i], acc ? (1u << j) : 0u);
   s.e = acc ? en : s.e;
-  s.b = acc ? s.b + (de >> c.shift) : s.b;
+  s.bw = acc ? bw_new : s.bw;
   s.g = acc ? g_new : s.g;
   const bool upd = valid & c.upd;
   s.g = upd ? s.g + c.lnf : s.g;
-  const int at = __mul24(s.b, c.W);
-  c.G[at] = s.g;  // unchanged value when !upd
-  atomicAdd(&c.H[at], upd ? 1u : 0u);
+  c.G[s.bw] = s.g;  // unchanged value when !upd
+  atomicAdd(&c.H[s.bw], upd ? 1u : 0u);
   s.bad_e = (s.alive & !valid) ? en : s.bad_e;
   s.alive = valid;
 }
@@ -210,6 +214,7 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
                                       int n)
 {
   int it = 0;
+  ws.bw = __mul24(ws.b, cx.W);
   uint64_t blk = att >> 1;
   philox4 cur = rng.eval(blk);
   if ((att & 1u) && it < n) {  // odd start: second half of the block
