@@ -204,20 +204,87 @@ __device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& 
   s.alive = valid;
 }
 
-// n attempts of one walker starting at attempt counter att: two attempts
-// per Philox block; the block of the next pair is computed, five rounds at
-// a time, while each attempt of the current pair waits for its reads.
-// Stops early (alive = false) at an energy outside the spectrum; att then
-// counts the attempts up to and including that one.
+// One pair of attempts (both words of a Philox block): [reads of attempt 1]
+// five rounds of the next block [rest of attempt 1] [reads of attempt 2]
+// five rounds [rest of attempt 2]; the fences keep each half of the next
+// block between an attempt's reads and their first use.  Returns the
+// attempts that counted: those made while the walker was alive.
+template<int BOUNDARY>
+__device__ __forceinline__ uint32_t attempt_pair(const sweep_ctx& cx, sweep_state& ws, const philox4& cur,
+                                                 philox_run& nx)
+{
+  // acceptance bands depend on the words only: off the attempts' chains
+  const accept_band b1 = accept_bounds(cur.v[1]);
+  const accept_band b2 = accept_bounds(cur.v[3]);
+  const uint32_t a1 = ws.alive ? 1u : 0u;
+  const attempt_reads r1 = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
+  nx.fence();
+  nx.rounds5();
+  nx.fence();
+  attempt_finish<BOUNDARY>(cx, ws, r1, cur.v[1], b1);
+  const uint32_t a2 = ws.alive ? 1u : 0u;  // the attempt that meets a new energy counts, later ones do not
+  const attempt_reads r2 = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
+  nx.fence();
+  nx.rounds5();
+  nx.fence();
+  attempt_finish<BOUNDARY>(cx, ws, r2, cur.v[3], b2);  // no effect once dead
+  return a1 + a2;
+}
+
+// n attempts of one walker starting at attempt counter att, two per Philox
+// block.  Stops counting (alive = false) at an energy outside the spectrum;
+// att then counts the attempts up to and including that one, and the
+// walker's later attempts have no effect.
+//
+// Every walker of a workgroup enters a round at the same attempt count (each
+// makes n attempts per round), so the block counter and the trip counts are
+// normally the same on every lane: the loop then runs wave-uniform, with no
+// per-lane exit or counter updates.  Lanes that disagree take the per-lane
+// loop below, which gives the same results.
 template<int BOUNDARY>
 __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, const attempt_stream& rng, uint64_t& att,
                                       int n)
 {
-  int it = 0;
   ws.bw = __mul24(ws.b, cx.W);
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint32_t lo0 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(att));
+  const uint32_t hi0 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(att >> 32));
+  const uint64_t att0 = (static_cast<uint64_t>(hi0) << 32) | lo0;
+#else
+  const uint64_t att0 = att;
+#endif
+  if (__all(att == att0)) {
+    uint32_t done = 0;
+    int left = n;
+    uint64_t blk = att0 >> 1;
+    philox4 cur = rng.eval(blk);
+    if ((att0 & 1u) && left > 0) {  // odd start: second half of the block
+      done += ws.alive ? 1u : 0u;
+      const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
+      attempt_finish<BOUNDARY>(cx, ws, r, cur.v[3], accept_bounds(cur.v[3]));
+      --left;
+      ++blk;
+      cur = rng.eval(blk);
+    }
+    for (; left >= 2; left -= 2) {
+      philox_run nx;
+      nx.begin(blk + 1, rng.slot, rng.key0);
+      done += attempt_pair<BOUNDARY>(cx, ws, cur, nx);
+      ++blk;
+      cur = philox4{{nx.c0, nx.c1, nx.c2, nx.c3}};
+    }
+    if (left > 0) {
+      done += ws.alive ? 1u : 0u;
+      const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
+      attempt_finish<BOUNDARY>(cx, ws, r, cur.v[1], accept_bounds(cur.v[1]));
+    }
+    att = att0 + done;
+    return;
+  }
+  int it = 0;
   uint64_t blk = att >> 1;
   philox4 cur = rng.eval(blk);
-  if ((att & 1u) && it < n) {  // odd start: second half of the block
+  if ((att & 1u) && it < n) {
     const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
     attempt_finish<BOUNDARY>(cx, ws, r, cur.v[3], accept_bounds(cur.v[3]));
     ++att;
@@ -226,26 +293,9 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
     cur = rng.eval(blk);
   }
   for (; it + 2 <= n && ws.alive; it += 2) {
-    // [reads of attempt 1] five rounds [rest of attempt 1] [reads of
-    // attempt 2] five rounds [rest of attempt 2]: the fences keep each half
-    // of the next block between an attempt's reads and their first use.
     philox_run nx;
     nx.begin(blk + 1, rng.slot, rng.key0);
-    // acceptance bands depend on the words only: off the attempts' chains
-    const accept_band b1 = accept_bounds(cur.v[1]);
-    const accept_band b2 = accept_bounds(cur.v[3]);
-    const attempt_reads r1 = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
-    nx.fence();
-    nx.rounds5();
-    nx.fence();
-    attempt_finish<BOUNDARY>(cx, ws, r1, cur.v[1], b1);
-    const bool first_ok = ws.alive;
-    const attempt_reads r2 = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
-    nx.fence();
-    nx.rounds5();
-    nx.fence();
-    attempt_finish<BOUNDARY>(cx, ws, r2, cur.v[3], b2);  // no effect once dead
-    att += first_ok ? 2 : 1;  // the attempt that met the new energy counts, later ones do not
+    att += attempt_pair<BOUNDARY>(cx, ws, cur, nx);
     ++blk;
     cur = philox4{{nx.c0, nx.c1, nx.c2, nx.c3}};
   }
