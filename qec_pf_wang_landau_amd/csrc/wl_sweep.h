@@ -60,19 +60,26 @@ struct sweep_ctx {
   double lnf;
 };
 
-// Philox4x32-10 of wl_math.h split into two halves of five rounds, so that
-// each half can fill the LDS shadow of one attempt.
+// Philox4x32-10 of wl_math.h (counter (block lo, block hi, slot, 0), key
+// (seed + sample, walk tag)), computed in three parts: the first two rounds
+// at once, where the block index is the same on every lane and M1 * slot is
+// fixed for the sweep (so most of their multiplies are scalar or hoisted),
+// then two halves of four rounds, each filling the LDS shadow of one
+// attempt.
 struct philox_run {
   uint32_t c0, c1, c2, c3, k0, k1;
 
-  __device__ __forceinline__ void begin(uint64_t block, uint32_t slot, uint32_t key0)
+  // Rounds 1 and 2 of the block `block`; ms_hi/ms_lo = M1 * slot.
+  __device__ __forceinline__ void begin2(uint64_t block, uint32_t ms_hi, uint32_t ms_lo, uint32_t key0)
   {
-    c0 = static_cast<uint32_t>(block);
-    c1 = static_cast<uint32_t>(block >> 32);
-    c2 = slot;
-    c3 = 0u;
-    k0 = key0;
-    k1 = kTagWalk;
+    const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * static_cast<uint32_t>(block);
+    c0 = ms_hi ^ static_cast<uint32_t>(block >> 32) ^ key0;
+    c1 = ms_lo;
+    c2 = static_cast<uint32_t>(p0 >> 32) ^ kTagWalk;
+    c3 = static_cast<uint32_t>(p0);
+    k0 = key0 + 0x9E3779B9u;
+    k1 = kTagWalk + 0xBB67AE85u;
+    rounds<1>();
   }
 
   // Scheduling fence: the block state is (re)defined here, so rounds after
@@ -80,10 +87,11 @@ struct philox_run {
   // it; as a volatile statement it also keeps LDS accesses on their side.
   __device__ __forceinline__ void fence() { __asm__ volatile("" : "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3)); }
 
-  __device__ __forceinline__ void rounds5()
+  template<int R>
+  __device__ __forceinline__ void rounds()
   {
 #pragma unroll
-    for (int r = 0; r < 5; ++r) {
+    for (int r = 0; r < R; ++r) {
       const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
       const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
       const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
@@ -192,7 +200,7 @@ __device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& 
   const int bw_new = s.bw + __mul24(de >> c.shift, c.W);  // off the decision's chain
   const bool valid = s.alive & flag;
   const bool acc = valid & in_range & accept_banded(s.g - g_new, xa, band);
-  atomicXor(&c.L[r.i], acc ? (1u << j) : 0u);
+  c.L[r.i] = r.rc ^ (acc ? (1u << j) : 0u);  // the row is this walker's alone: no atomic needed
   s.e = acc ? en : s.e;
   s.bw = acc ? bw_new : s.bw;
   s.g = acc ? g_new : s.g;
@@ -213,19 +221,20 @@ template<int BOUNDARY>
 __device__ __forceinline__ uint32_t attempt_pair(const sweep_ctx& cx, sweep_state& ws, const philox4& cur,
                                                  philox_run& nx)
 {
+  // nx holds the next block after its first two rounds
   // acceptance bands depend on the words only: off the attempts' chains
   const accept_band b1 = accept_bounds(cur.v[1]);
   const accept_band b2 = accept_bounds(cur.v[3]);
   const uint32_t a1 = ws.alive ? 1u : 0u;
   const attempt_reads r1 = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
   nx.fence();
-  nx.rounds5();
+  nx.rounds<4>();
   nx.fence();
   attempt_finish<BOUNDARY>(cx, ws, r1, cur.v[1], b1);
   const uint32_t a2 = ws.alive ? 1u : 0u;  // the attempt that meets a new energy counts, later ones do not
   const attempt_reads r2 = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
   nx.fence();
-  nx.rounds5();
+  nx.rounds<4>();
   nx.fence();
   attempt_finish<BOUNDARY>(cx, ws, r2, cur.v[3], b2);  // no effect once dead
   return a1 + a2;
@@ -246,6 +255,8 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
                                       int n)
 {
   ws.bw = __mul24(ws.b, cx.W);
+  const uint64_t ms = static_cast<uint64_t>(0xCD9E8D57u) * rng.slot;  // round 1's M1 * slot, fixed per walker
+  const uint32_t ms_hi = static_cast<uint32_t>(ms >> 32), ms_lo = static_cast<uint32_t>(ms);
 #if defined(__HIP_DEVICE_COMPILE__)
   const uint32_t lo0 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(att));
   const uint32_t hi0 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(att >> 32));
@@ -268,7 +279,7 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
     }
     for (; left >= 2; left -= 2) {
       philox_run nx;
-      nx.begin(blk + 1, rng.slot, rng.key0);
+      nx.begin2(blk + 1, ms_hi, ms_lo, rng.key0);
       done += attempt_pair<BOUNDARY>(cx, ws, cur, nx);
       ++blk;
       cur = philox4{{nx.c0, nx.c1, nx.c2, nx.c3}};
@@ -294,7 +305,7 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
   }
   for (; it + 2 <= n && ws.alive; it += 2) {
     philox_run nx;
-    nx.begin(blk + 1, rng.slot, rng.key0);
+    nx.begin2(blk + 1, ms_hi, ms_lo, rng.key0);
     att += attempt_pair<BOUNDARY>(cx, ws, cur, nx);
     ++blk;
     cur = philox4{{nx.c0, nx.c1, nx.c2, nx.c3}};
