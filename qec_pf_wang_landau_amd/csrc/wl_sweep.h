@@ -255,7 +255,8 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
                                       int n)
 {
   ws.bw = __mul24(ws.b, cx.W);
-  const uint64_t ms = static_cast<uint64_t>(0xCD9E8D57u) * rng.slot;  // round 1's M1 * slot, fixed per walker
+  // M1 * slot of the first Philox round: fixed per walker
+  const uint64_t ms = static_cast<uint64_t>(0xCD9E8D57u) * rng.slot;
   const uint32_t ms_hi = static_cast<uint32_t>(ms >> 32), ms_lo = static_cast<uint32_t>(ms);
 #if defined(__HIP_DEVICE_COMPILE__)
   const uint32_t lo0 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(att));
