@@ -49,13 +49,16 @@ def parse():
     ap.add_argument("--exact", action="store_true",
                     help="6x6: after timing, enumerate the exact DOS of every task on the GPU and report the "
                          "decoded ln g error and the agreement of the class decisions")
-    ap.add_argument("--workload", choices=("6x6", "16x16", "32x32", "serve", "uneven"), default="6x6",
+    ap.add_argument("--workload", choices=("6x6", "16x16", "32x32", "serve", "uneven", "curve"), default="6x6",
                     help="6x6: the 400-task batch (default); 16x16: periodic p=0.1 samples, 64 intervals x 64 "
                          "walkers, per-interval batched path vs the plain loop, both to ln f=1e-6; 32x32: one "
                          "p=0.1 task, 64 intervals x 128 walkers, attempts/s over a bounded number of rounds")
     ap.add_argument("--samples", type=int, default=None,
                     help="disorder samples per class (default 100 for 6x6, 1 for 16x16)")
     ap.add_argument("--rounds", type=int, default=2000, help="rounds timed by the 32x32 workload")
+    ap.add_argument("--curve-L", type=int, nargs="+", default=[4, 6], help="curve: lattice sizes")
+    ap.add_argument("--curve-p", type=float, nargs="+", default=[0.04, 0.07, 0.10, 0.13],
+                    help="curve: flip probabilities")
     ap.add_argument("--prerun-iters", type=int, default=1000000, help="prerun steps for 16x16 and 32x32")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -424,6 +427,56 @@ def bench_uneven(args, engine, torch) -> dict:
     }
 
 
+def bench_curve(args, engine, torch) -> dict:
+    """Logical error curve: per (L, p) the decoded samples' failure rate with a
+    Wilson 95 % interval, from Wang-Landau (timed: couplings, prerun, run
+    to ln f = 1e-6, device stitching and ln Z) and from exact enumeration of
+    the same samples (L^2 <= 40), and the agreement of the two decisions."""
+    from qec_pf_wang_landau_amd import error_rate  # noqa: PLC0415
+    S = args.samples or 200
+    Ls, ps = args.curve_L, args.curve_p
+    error_rate.decode_point(Ls[0], ps[0], 4, error_rate.run_config(Ls[0], seed=1))  # warmup
+    points, total_s, decoded, attempts = [], 0.0, 0, 0
+    for L in Ls:
+        cfg = error_rate.run_config(L, seed=12345)
+        for prob in ps:
+            wl = error_rate.decode_point(L, prob, S, cfg)
+            total_s += wl.seconds
+            decoded += wl.decoded
+            attempts += wl.attempts
+            row = {"L": L, "p": prob, "wl": {"rate": wl.rate, "wilson": [wl.lo, wl.hi], "decoded": wl.decoded,
+                                             "seconds": wl.seconds, "rounds_max": wl.extra.get("rounds_max")}}
+            if L * L <= 40:
+                ex = error_rate.decode_point(L, prob, S, cfg, method="exact")
+                srt = np.sort(ex.log_z, axis=1)
+                clear = (srt[:, -1] - srt[:, -2]) > 0.5
+                ok = wl.decision >= 0
+                row["exact"] = {"rate": ex.rate, "wilson": [ex.lo, ex.hi], "seconds": ex.seconds}
+                row["decisions_agree"] = float(np.mean(wl.decision[ok] == ex.decision[ok]))
+                row["decisions_agree_where_gap_gt_0.5"] = (float(np.mean(wl.decision[ok & clear]
+                                                                         == ex.decision[ok & clear]))
+                                                           if (ok & clear).any() else None)
+            points.append(row)
+            log(f"curve L={L} p={prob}: wl rate {wl.rate:.4f} [{wl.lo:.4f}, {wl.hi:.4f}] in {wl.seconds:.1f}s"
+                + (f"; exact {row['exact']['rate']:.4f}, agree {row['decisions_agree']:.3f}" if "exact" in row
+                   else ""))
+    achieved = attempts * BYTES_PER_ATTEMPT / total_s / 1e9
+    return {
+        "metric": METRIC, "value": decoded / total_s, "unit": "samples/s", "n_gpus": 1, "steps": len(points),
+        "warmup": 1, "ms_per_step": total_s / len(points) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"logical error curve: L in {Ls}, p in {ps}, {S} samples per point from seed 12345, "
+                               "4 classes each, periodic, ln f 1 -> 1e-6, (I, W) per L from error_rate.DEFAULT_SIZES; "
+                               "timed per point: couplings, prerun, Wang-Landau, device stitching and ln Z",
+                   "samples_per_point": S},
+        "points": points,
+        "attempts_per_s": attempts / total_s,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
 def exact_check(engine, torch, dev, batch, samples: int) -> dict:
     """Gray-code exact DOS of every task (GPU) against the decoded ln g of the
     timed run: max/mean abs error, and the class decision at the Nishimori
@@ -469,7 +522,8 @@ def main():
 
     torch.cuda.set_device(0)
     if args.workload != "6x6":
-        fn = {"16x16": bench_16x16, "32x32": bench_32x32, "serve": bench_serve, "uneven": bench_uneven}[args.workload]
+        fn = {"16x16": bench_16x16, "32x32": bench_32x32, "serve": bench_serve, "uneven": bench_uneven,
+              "curve": bench_curve}[args.workload]
         print(json.dumps(fn(args, engine, torch)))
         return
     args.samples = args.samples or 100

@@ -46,6 +46,19 @@ double wl_log_partition(const int* E, const double* lng, int n, double beta);
 /* Nishimori inverse temperature 1/2 ln((1-p)/p). */
 double wl_nishimori_beta(double p);
 
+/* Logical-class decision of num_samples samples from their ln Z
+ * (num_samples x num_classes, class 0 = 'i', the sampled error's own coset).
+ * decision[s] = first argmax (-1 if any ln Z is not finite); fail_weight[s] =
+ * probability that a decoder choosing uniformly among the tied maxima does
+ * not pick class 0 (0 or 1 without ties; NaN when undecided).  No counterpart
+ * in the reference: the aggregation after decoding that README.md:198-200
+ * cites (logical error curves). */
+int wl_decide_classes(const double* log_z, int num_samples, int num_classes, int* decision, double* fail_weight);
+
+/* Wilson score interval [lo, hi] of the failure rate failures/n at z
+ * standard errors (z = 1.96 for 95 %).  No counterpart in the reference. */
+int wl_wilson_interval(double failures, int n, double z, double* lo, double* hi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "wl_stitch_log_g": (I32, [P, P, P, I32, I32, I32, I32, P, P, I32, P]),
     "wl_log_partition": (F64, [P, P, I32, F64]),
     "wl_nishimori_beta": (F64, [F64]),
+    "wl_decide_classes": (I32, [P, I32, I32, P, P]),
+    "wl_wilson_interval": (I32, [F64, I32, F64, P, P]),
 }
 
 

@@ -28,7 +28,8 @@ ARCH = os.environ.get("WL_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["wl_plain.hip", "wl_fused.hip", "wl_block.hip", "wl_decode.hip", "wl_exact.hip"]
 HOST_SOURCES = ["wl_host.cpp"]
-EXECUTABLES = {"prerun": "prerun_main.cpp", "mainrun": "mainrun_main.cpp", "results_dump": "results_dump.cpp"}
+EXECUTABLES = {"prerun": "prerun_main.cpp", "mainrun": "mainrun_main.cpp", "results_dump": "results_dump.cpp",
+               "decode_results": "decode_results.cpp"}
 CLI_SOURCES = ["wl_io.cpp"]
 
 COMMON = [

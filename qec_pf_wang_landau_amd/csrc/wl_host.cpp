@@ -190,4 +190,56 @@ double wl_nishimori_beta(double p)
   return 0.5 * wlamd::wl_log((1.0 - p) / p);
 }
 
+// Class 0 ('i') is the coset of the sampled error itself, so a sample fails
+// unless class 0 is the most likely class.  Among m tied maxima the decoder
+// picks uniformly, which fails with probability 1 - 1/m when class 0 is one
+// of them.
+int wl_decide_classes(const double* log_z, int num_samples, int num_classes, int* decision, double* fail_weight)
+{
+  return guarded_host([&] {
+    if (num_samples < 0 || num_classes < 1) {
+      throw std::invalid_argument("decide_classes: need num_samples >= 0 and num_classes >= 1");
+    }
+    for (int s = 0; s < num_samples; ++s) {
+      const double* z = log_z + static_cast<int64_t>(s) * num_classes;
+      bool finite = true;
+      int best = 0;
+      for (int c = 0; c < num_classes; ++c) {
+        finite = finite && std::isfinite(z[c]);
+        if (z[c] > z[best]) {
+          best = c;
+        }
+      }
+      if (!finite) {
+        decision[s] = -1;
+        fail_weight[s] = std::nan("");
+        continue;
+      }
+      int ties = 0;
+      for (int c = 0; c < num_classes; ++c) {
+        ties += z[c] == z[best] ? 1 : 0;
+      }
+      decision[s] = best;
+      fail_weight[s] = z[0] == z[best] ? 1.0 - 1.0 / static_cast<double>(ties) : 1.0;
+    }
+  });
+}
+
+// Wilson score interval of a binomial proportion k/n at z standard errors:
+// (k + z^2/2 -+ z sqrt(k (n-k)/n + z^2/4)) / (n + z^2).
+int wl_wilson_interval(double failures, int n, double z, double* lo, double* hi)
+{
+  return guarded_host([&] {
+    if (n < 1 || failures < 0.0 || failures > static_cast<double>(n) || !(z > 0.0)) {
+      throw std::invalid_argument("wilson_interval: need n >= 1, 0 <= failures <= n and z > 0");
+    }
+    const double nn = static_cast<double>(n);
+    const double z2 = z * z;
+    const double centre = failures + 0.5 * z2;
+    const double half = z * std::sqrt(failures * (nn - failures) / nn + 0.25 * z2);
+    *lo = std::max(0.0, (centre - half) / (nn + z2));
+    *hi = std::min(1.0, (centre + half) / (nn + z2));
+  });
+}
+
 }  // extern "C"
