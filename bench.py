@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--curve-p", type=float, nargs="+", default=[0.04, 0.07, 0.10, 0.13],
                     help="curve: flip probabilities")
     ap.add_argument("--prerun-iters", type=int, default=1000000, help="prerun steps for 16x16 and 32x32")
+    ap.add_argument("--prob", type=float, default=0.1, help="32x32: flip probability (0 = the clean torus)")
+    ap.add_argument("--converge", action="store_true",
+                    help="32x32: run the task to ln f <= 1e-8 and report the time to solution")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-tasks", type=int, default=16)
@@ -158,12 +161,13 @@ def copy_probe(args, torch) -> None:
     del src, dst
 
 
-def large_problem(engine, X: int, I: int, W: int, S: int, classes, beta: float, prerun_iters: int):
-    """Periodic p = 0.1 samples with the prerun on the GPU (4 I prerun
-    intervals so that every mainrun interval gets a seed lattice)."""
+def large_problem(engine, X: int, I: int, W: int, S: int, classes, beta: float, prerun_iters: int,
+                  prob: float = 0.1):
+    """Periodic samples at flip probability prob with the prerun on the GPU
+    (4 I prerun intervals so that every mainrun interval gets a seed lattice)."""
     cfg = engine.RunConfig(X=X, Y=X, boundary=0, num_intervals=I, walker_per_interval=W, overlap=0.25, alpha=0.8,
                            beta=beta, num_iterations=1000, exchange_every=20, seed=12345)
-    per_class = {c: engine.problem_gpu(cfg, S, 0.1, c, prerun_walkers=4 * I, prerun_iters=prerun_iters,
+    per_class = {c: engine.problem_gpu(cfg, S, prob, c, prerun_walkers=4 * I, prerun_iters=prerun_iters,
                                        prerun_intervals=4 * I) for c in classes}
     states = [engine.build_state(per_class[c], cfg) for c in classes]
     return cfg, states
@@ -258,11 +262,46 @@ def bench_16x16(args, engine, torch) -> dict:
     }
 
 
+def bench_32x32_converge(args, engine, torch, cfg, states) -> dict:
+    """The large-lattice task run to ln f <= 1e-8: time to solution."""
+    S = states[0].S
+    dev = engine.BatchDevice(engine.merge_states(states))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while True:
+        dev.run_blocks(20000, 64, True)
+        status = engine.to_host(dev.t["status"], np.int32)
+        rounds = engine.to_host(dev.t["rounds"], np.int64)
+        log(f"32x32 converge: {time.perf_counter() - t0:.1f}s, rounds {rounds.tolist()}, status {status.tolist()}")
+        if (status != 0).all():
+            break
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    attempts = int(rounds.sum()) * 64 * 128 * cfg.num_iterations
+    done = int((status == 1).sum())
+    achieved = attempts * BYTES_PER_ATTEMPT / dt / 1e9
+    return {
+        "metric": METRIC, "value": done / dt, "unit": "samples/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"32x32 periodic RBIM p={args.prob}, {S} task(s) class i, 64 intervals overlap "
+                               "0.25, 128 walkers/interval (8192 walkers), ln f 1 -> 1e-8, exchange every 20 "
+                               "rounds, run to convergence; per-interval batched path (hipGraph blocks)",
+                   "rounds": rounds.tolist()},
+        "seconds_to_converge": dt, "tasks_finished": done, "attempts_per_s": attempts / dt,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
 def bench_32x32(args, engine, torch) -> dict:
     S = args.samples or 1
     t_setup = time.perf_counter()
-    cfg, states = large_problem(engine, 32, 64, 128, S, ("i",), 1e-8, args.prerun_iters)
+    cfg, states = large_problem(engine, 32, 64, 128, S, ("i",), 1e-8, args.prerun_iters, args.prob)
     log(f"32x32 setup (disorder + prerun) {time.perf_counter() - t_setup:.1f}s")
+    if args.converge:
+        return bench_32x32_converge(args, engine, torch, cfg, states)
     dev = engine.BatchDevice(engine.merge_states(states))
     dev.run_blocks(20, 64, True)  # warmup: first block and graph build
     torch.cuda.synchronize()
@@ -283,8 +322,8 @@ def bench_32x32(args, engine, torch) -> dict:
         "metric": METRIC, "value": attempts / dt, "unit": "attempts/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"32x32 periodic RBIM p=0.1, {S} task(s) class i, 64 intervals overlap 0.25, 128 "
-                               f"walkers/interval (8192 walkers), {args.rounds} rounds of 1000 attempts/walker, "
+        "config": {"workload": f"32x32 periodic RBIM p={args.prob}, {S} task(s) class i, 64 intervals overlap 0.25, "
+                               f"128 walkers/interval (8192 walkers), {args.rounds} rounds of 1000 attempts/walker, "
                                "exchange every 20 rounds; per-interval batched path (hipGraph blocks)",
                    "rounds": rounds},
         "ns_per_round": dt / max(rounds, 1) * 1e9,

@@ -59,6 +59,7 @@ SIG = {
     "wlo_max_threads": (I32, []),
     "wlo_exact_dos_bruteforce": (I32, [P, I32, I32, I32, P]),
     "wlo_exact_dos_transfer": (I32, [P, I32, I32, I32, P, P]),
+    "wlo_kaufman_dos": (I32, [I32, I32, P]),
 }
 
 _O = None
@@ -107,6 +108,16 @@ def exact_lng(J: np.ndarray, X: int, Y: int, boundary: int) -> np.ndarray:
     j = np.ascontiguousarray(J, np.int8)
     rc = oracle().wlo_exact_dos_transfer(p(j), X, Y, boundary, p(lng), p(cnt))
     assert rc == 0
+    return lng
+
+
+def kaufman_lng(X: int, Y: int) -> np.ndarray:
+    """Exact ln g(E) over [-2N, 2N] of the all-ferromagnetic X x Y torus (Y
+    even) from Kaufman's partition function."""
+    N = X * Y
+    lng = np.zeros(4 * N + 1)
+    rc = oracle().wlo_kaufman_dos(X, Y, p(lng))
+    assert rc == 0, rc
     return lng
 
 

@@ -23,6 +23,10 @@ Parity status per oracle function (what pins it):
   stitch               invariants integration_mainrun.py:118 (LSE = N ln 2) and energy set == spectrum;
                        ln g vs the exact DOS after a full run
   exact DOS (transfer) brute-force enumeration on small lattices, sum g = 2^N, g(E) = g(-E) when bipartite
+  exact DOS (Kaufman)  the transfer matrix on 12 torus shapes (3..16 rows x 4 or 6 columns, odd rows
+                       included); at 16x16 and 32x32: sum g = 2^N, g(E) = g(-E), and the counts of the
+                       four lowest levels (2, 2N, 4N, N(N-1) + 10N from flipping one spin, an adjacent
+                       pair, two separate spins, a straight or bent trimer or a 2x2 plaquette)
   log_partition        closed form on the exact DOS (Z(0) = 2^N) and direct summation
 """
 
@@ -34,7 +38,7 @@ import numpy as np
 import pytest
 
 from helpers import exact_samples, oracle_disorder, oracle_prerun
-from oracle_lib import exact_lng, oracle, p, ref, run_loop
+from oracle_lib import exact_lng, kaufman_lng, oracle, p, ref, run_loop
 from qec_pf_wang_landau_amd import engine
 
 
@@ -607,3 +611,41 @@ def test_mainrun_c1_converges_to_exact_dos():
     exact = exact_lng(samples[0].J, 4, 4, 0)[E + 32]
     err = np.max(np.abs(lng - exact))
     assert err < 0.05, err
+
+
+@pytest.mark.parametrize("X,Y", [(3, 4), (4, 4), (5, 4), (6, 4), (8, 4), (16, 4), (3, 6), (4, 6), (6, 6), (7, 6),
+                                 (8, 6), (10, 6)])
+def test_kaufman_matches_transfer_matrix(X, Y):
+    """Kaufman's torus partition function, expanded into g(E), against the
+    transfer matrix on the all-ferromagnetic torus (same support, ln g to
+    rounding)."""
+    N = X * Y
+    tm = exact_lng(np.ones(2 * N, np.int8), X, Y, 0)
+    kf = kaufman_lng(X, Y)
+    assert (np.isfinite(tm) == np.isfinite(kf)).all()
+    fin = np.isfinite(tm)
+    assert np.max(np.abs(tm[fin] - kf[fin])) < 1e-12
+
+
+@pytest.mark.parametrize("L", [16, 32])
+def test_kaufman_large_tori_invariants(L):
+    N = L * L
+    lng = kaufman_lng(L, L)
+    fin = np.isfinite(lng)
+    m = lng[fin].max()
+    assert m + math.log(np.exp(lng[fin] - m).sum()) == pytest.approx(N * math.log(2.0), abs=1e-11)
+    assert np.array_equal(fin, fin[::-1]) and np.max(np.abs(lng[fin] - lng[::-1][fin])) < 1e-11
+    # E = -2N + 2u with u unsatisfied bonds: u = 0, 4, 6, 8 are the four lowest levels, u = 2 is empty
+    E0 = 0
+    assert np.rint(np.exp(lng[[E0, E0 + 8, E0 + 12]])).tolist() == [2.0, 2.0 * N, 4.0 * N]
+    assert round(math.exp(lng[E0 + 16])) == N * (N - 1) + 10 * N
+    assert not fin[E0 + 4] and not fin[1::2].any()
+    # energies step by 4 and the spectrum is [-2N, 2N] without the two gaps next to the ends
+    support = np.flatnonzero(fin) - 2 * N
+    assert support.tolist() == [-2 * N] + list(range(-2 * N + 8, 2 * N - 7, 4)) + [2 * N]
+
+
+def test_kaufman_rejects_unsupported_shapes():
+    out = np.zeros(4 * 25 + 1)
+    for X, Y in ((5, 5), (2, 4), (4, 2)):
+        assert oracle().wlo_kaufman_dos(X, Y, p(out)) == -1
