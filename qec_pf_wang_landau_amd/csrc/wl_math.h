@@ -355,7 +355,7 @@ WL_HD bool accept_exact(double d, uint32_t x)
 // Same decision as accept_exact, faster on the device, in the log domain:
 // u < exp(d) iff ln u < d.  ln u is estimated in FP32 from the word alone
 // (hardware log2 of the FP32 u, times ln 2), so it can be computed before d
-// is known; d is compared in FP32 against ln u +- 2^-14.  Error of the
+// is known; d is compared against the bounds ln u +- 2^-14.  Error of the
 // estimate for u in [2^-33, 1): FP32 u within 2^-23 relative (ln error <
 // 2^-22), log2 within 2 ulp of a result of magnitude < 33 (< 2^-17), the
 // ln 2 product 1 ulp (< 2^-19); d itself is compared in FP64 against the
@@ -377,17 +377,37 @@ __device__ __forceinline__ accept_band accept_bounds(uint32_t x)
   return accept_band{lu - 0x1p-14f, lu + 0x1p-14f};
 }
 
-__device__ __forceinline__ bool accept_banded(double d, uint32_t x, accept_band band)
+// The band as two FP64 limits, computed off the chain: accept iff d > t,
+// and d in [lo, t] takes the exact rule.  d >= 0 or d > hi accepts, so
+// t = hi when hi < 0 and otherwise the largest negative double (d > t is
+// then d >= 0); the rejections d < lo stay rejections.
+struct accept_limits {
+  double t, lo;
+};
+
+__device__ __forceinline__ accept_limits accept_limits_of(accept_band band)
 {
-  // the bounds widen to FP64 off the chain; d is compared without rounding
-  const bool pos = d >= 0.0;
-  const bool fa = d > static_cast<double>(band.hi);
-  const bool fr = d < static_cast<double>(band.lo);
-  bool acc = pos | fa;
-  if (!pos & !fa & !fr) {
-    acc = u32_to_unit(x) < wl_exp(d);
+  return accept_limits{(band.hi < 0.0f) ? static_cast<double>(band.hi) : -0x1p-1074,
+                       static_cast<double>(band.lo)};
+}
+
+// The decision from the limits: one FP64 compare on the attempt's chain.  The
+// exact rule sits behind a wave-uniform branch, so the common path changes
+// no execution mask.  Limits of +inf reject without the exact rule.
+__device__ __forceinline__ bool accept_limited(double d, uint32_t x, accept_limits lim)
+{
+  bool acc = d > lim.t;
+  const bool inb = (d >= lim.lo) & (d <= lim.t);
+  if (__builtin_amdgcn_ballot_w64(inb) != 0) {
+    const bool ex = u32_to_unit(x) < wl_exp(d);
+    acc = inb ? ex : acc;
   }
   return acc;
+}
+
+__device__ __forceinline__ bool accept_banded(double d, uint32_t x, accept_band band)
+{
+  return accept_limited(d, x, accept_limits_of(band));
 }
 
 __device__ __forceinline__ bool accept_fast(double d, uint32_t x)

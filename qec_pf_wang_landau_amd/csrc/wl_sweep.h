@@ -199,7 +199,14 @@ __device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& 
   g_new = (de == 0) ? s.g : g_new;
   const int bw_new = s.bw + __mul24(de >> c.shift, c.W);  // off the decision's chain
   const bool valid = s.alive & flag;
-  const bool acc = valid & in_range & accept_banded(s.g - g_new, xa, band);
+  // A move that is not valid or leaves the interval gets limits of +inf
+  // (never accepted, never the exact rule), so the decision is the one FP64
+  // compare of accept_limited, with no mask logic after it.
+  accept_limits lim = accept_limits_of(band);
+  const bool ok = valid & in_range;
+  lim.t = ok ? lim.t : __builtin_inf();
+  lim.lo = ok ? lim.lo : __builtin_inf();
+  const bool acc = accept_limited(s.g - g_new, xa, lim);
   c.L[r.i] = r.rc ^ (acc ? (1u << j) : 0u);  // the row is this walker's alone: no atomic needed
   s.e = acc ? en : s.e;
   s.bw = acc ? bw_new : s.bw;
