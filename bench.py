@@ -65,6 +65,8 @@ def parse():
                     help="32x32: run the task to ln f <= 1e-8 and report the time to solution")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip measuring the LDS read and HBM copy bandwidth after the timed steps")
     ap.add_argument("--cpu-tasks", type=int, default=16)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after timing, run one device copy of this many MiB (the reference kernel that scales "
@@ -159,6 +161,27 @@ def copy_probe(args, torch) -> None:
     torch.cuda.synchronize()
     log(f"copy probe: {n} bytes")
     del src, dst
+
+
+def measured_peaks(engine, torch) -> dict:
+    """LDS read bandwidth (wl_probe_lds_read) and HBM copy bandwidth (a 1 GiB
+    device copy, read + write bytes) measured on this device, best of 3."""
+    n = 1 << 30
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    best = 0.0
+    for it in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        if it > 0:
+            best = max(best, 2 * n / (e0.elapsed_time(e1) * 1e-3))
+    del src, dst
+    lds = engine.measure_lds_read_peak()
+    log(f"measured peaks: LDS read {lds / 1e12:.1f} TB/s, HBM copy {best / 1e12:.2f} TB/s")
+    return {"lds_read_TBs": lds / 1e12, "hbm_copy_TBs": best / 1e12}
 
 
 def large_problem(engine, X: int, I: int, W: int, S: int, classes, beta: float, prerun_iters: int,
@@ -630,6 +653,7 @@ def main():
         print(f"[bench] cpu: {cpu['value']:.3f} samples/s on {cpu['cores']} threads ({cpu['seconds']:.1f} s)",
               file=sys.stderr, flush=True)
 
+    peaks = None if args.no_peaks else measured_peaks(engine, torch)
     achieved = attempts * BYTES_PER_ATTEMPT / kern_s / 1e9
     out = {
         "metric": METRIC,
@@ -671,6 +695,11 @@ def main():
         "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None,
         "exact_check": exact,
     }
+    if peaks:
+        # the same algorithmic bytes against the LDS, where the batched kernel keeps them
+        out["measured_peaks"] = peaks
+        out["lds_roofline"] = {"achieved": achieved, "peak": peaks["lds_read_TBs"] * 1e3, "unit": "GB/s",
+                               "frac": achieved / (peaks["lds_read_TBs"] * 1e3)}
     print(json.dumps(out))
 
 

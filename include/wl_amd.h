@@ -296,6 +296,15 @@ size_t wl_block_interval_lds_bytes(const wl_task* task, const int* h_start, cons
  * reference. */
 int wl_exact_dos(const int8_t* d_J, int X, int Y, int boundary, int num_samples, uint64_t* d_counts, void* stream);
 
+/* LDS read-bandwidth probe: `blocks` workgroups of 256 threads stream
+ * reps x 8 reads of 16 bytes each from LDS (ds_read_b128) and write one word
+ * per thread to d_sink (blocks x 256 u32).  wl_probe_lds_read_bytes gives the
+ * bytes read; the caller times the launch on `stream`.  No counterpart in the
+ * reference: it measures the device's LDS peak, against which the batched
+ * kernels' per-attempt LDS traffic is compared. */
+int wl_probe_lds_read(int blocks, int reps, uint32_t* d_sink, void* stream);
+double wl_probe_lds_read_bytes(int blocks, int reps);
+
 /* Workspace bytes wl_stitch_tasks needs for one task with I intervals whose
  * lengths sum to total_len. */
 size_t wl_stitch_work_bytes(int num_intervals, int total_len);

@@ -767,3 +767,24 @@ def exact_lng_gpu(J: np.ndarray, X: int, Y: int, boundary: int) -> np.ndarray:
     c = exact_counts_gpu(J, X, Y, boundary).astype(np.float64)
     with np.errstate(divide="ignore"):
         return np.log(c)
+
+
+# ------------------------------------------------------- measured peaks --
+
+def measure_lds_read_peak(blocks: int = 4096, reps: int = 4000, repeat: int = 3) -> float:
+    """LDS read bandwidth of the device in bytes/s (wl_probe_lds_read, best
+    of `repeat` timed launches after one warmup), timed with HIP events on
+    the current stream."""
+    torch = _torch()
+    sink = torch.zeros(blocks * 256, dtype=torch.int32, device="cuda")
+    nbytes = float(lib().wl_probe_lds_read_bytes(blocks, reps))
+    best = 0.0
+    for it in range(repeat + 1):
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        call("wl_probe_lds_read", blocks, reps, dptr(sink), _cur_stream())
+        ev1.record()
+        ev1.synchronize()
+        if it > 0:
+            best = max(best, nbytes / (ev0.elapsed_time(ev1) * 1e-3))
+    return best

@@ -11,7 +11,7 @@ import pytest
 
 from helpers import compare_tasks, make_state, oracle_per_task, oracle_samples
 from qec_pf_wang_landau_amd import engine
-from qec_pf_wang_landau_amd._lib import call
+from qec_pf_wang_landau_amd._lib import WLError, call
 
 pytestmark = pytest.mark.gpu
 
@@ -92,3 +92,12 @@ def test_oversized_interval_is_refused(cuda):
     dev = engine.BatchDevice(engine.merge_states([st]))
     with pytest.raises(Exception, match="does not fit in LDS"):
         dev.run_blocks(4)
+
+
+def test_lds_read_probe_measures_a_plausible_peak(cuda):
+    """wl_probe_lds_read writes its sink and streams LDS at more than 1 TB/s
+    (the LDS of 256 CUs delivers tens of TB/s); bad arguments are refused."""
+    bw = engine.measure_lds_read_peak(blocks=1024, reps=200, repeat=1)
+    assert 1e12 < bw < 1e15
+    with pytest.raises(WLError):
+        call("wl_probe_lds_read", 0, 10, None, None)
