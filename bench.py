@@ -375,7 +375,7 @@ def bench_serve(args, engine, torch) -> dict:
         if lat.size:
             by_size[str(L)] = {"n": int(lat.size), "p50_s": float(np.percentile(lat, 50)),
                                "p99_s": float(np.percentile(lat, 99))}
-    log(f"serve: {summ} in {wall:.1f}s, {srv.slices} slices")
+    log(f"serve: {summ} in {wall:.1f}s, {srv.slices} slices; loop {srv.stats}")
     return {
         "metric": METRIC, "value": summ.get("decoded", 0) / wall, "unit": "decodes/s", "n_gpus": 1, "steps": 1,
         "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -385,6 +385,8 @@ def bench_serve(args, engine, torch) -> dict:
                                "(I, W) per L = " + str(serve.SIZES) + ", slices of 200 rounds, HBM budget "
                                f"{args.hbm_gib} GiB", "requests": len(reqs)},
         "latency": summ, "latency_by_size": by_size, "slices": srv.slices,
+        "loop_stats": {k: (v if not isinstance(v, dict) else {str(a): b for a, b in v.items()})
+                       for k, v in srv.stats.items()},
         "roofline": None, "cpu_baseline": None,
     }
 

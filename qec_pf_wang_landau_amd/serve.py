@@ -169,6 +169,10 @@ class Server:
         self.waiting: list[Request] = []
         self.results: list[Result] = []
         self.slices = 0
+        # host seconds per phase of the loop, device seconds of the slices,
+        # and slices per lattice size
+        self.stats = {"host_launch_s": 0.0, "host_admit_s": 0.0, "host_collect_s": 0.0, "host_decode_s": 0.0,
+                      "slice_gpu_s": 0.0, "slices_by_size": {L: 0 for L in self.sizes}}
 
     # ---------------------------------------------------------------- sizes --
     def config(self, L: int) -> engine.RunConfig:
@@ -316,18 +320,22 @@ class Server:
         torch.cuda.current_stream().synchronize()  # uploads complete before the slice reads them
         d = {k: engine.dptr(v) for k, v in self.t.items()}
         stream = self.streams[L]
+        start = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
         call("wl_fused_launch", engine._p(plan), engine.dptr(d_tasks), d["lattice"], d["J"], d["H"], d["logG"],
              d["factor"], d["E"], d["off_iter"], d["off_lattice"], d["idx"], d["cond"],
              engine.dptr(d_small["cond_sample"]), d["start"], d["end"], d["spectrum"],
              engine.dptr(d_small["counter"]), engine.dptr(d_small["rounds"]), engine.dptr(d_small["status"]),
              engine.dptr(self.queues[L]), None, self.slice_rounds, C.c_void_p(stream.cuda_stream))
-        done = torch.cuda.Event()
+        done = torch.cuda.Event(enable_timing=True)
         done.record(stream)
-        return d_tasks, d_small, done
+        return d_tasks, d_small, done, start
 
     def _collect(self, L: int) -> None:
-        owners, d_tasks, d_small, _ = self.inflight.pop(L)
+        owners, d_tasks, d_small, done, start = self.inflight.pop(L)
         self.streams[L].synchronize()
+        self.stats["slice_gpu_s"] += start.elapsed_time(done) * 1e-3
+        self.stats["slices_by_size"][L] += 1
         small = {k: engine.to_host(v, np.int64 if k in ("counter", "rounds") else np.int32)
                  for k, v in d_small.items()}
         for j, (lv, q) in enumerate(owners):
@@ -345,6 +353,7 @@ class Server:
     def _decode(self, done: list[_Live]) -> None:
         """Stitch + ln Z on the device for the finished requests' tasks."""
         torch = self.torch
+        t0 = time.perf_counter()
         tasks = np.concatenate([lv.tasks for lv in done])
         status = np.concatenate([lv.small["status"] for lv in done])
         cond_sample = np.where(status == 1, -1, 0).astype(np.int32)
@@ -383,6 +392,7 @@ class Server:
             self.results.append(Result(lv.req.rid, lv.req.L, z, int(np.argmax(z)), G, E, lv.small["rounds"].copy(),
                                        lv.req.arrival, lv.admitted, now, bad[0] if bad else 1))
             self._release(lv)
+        self.stats["host_decode_s"] += time.perf_counter() - t0
 
     # ----------------------------------------------------------------- loop --
     def step(self, now: float) -> bool:
@@ -390,6 +400,7 @@ class Server:
         admit arrivals while slices run, collect the slices that ended.
         Returns False when nothing ended or was launched."""
         progressed = False
+        t0 = time.perf_counter()
         for L in self.sizes:
             if L in self.inflight:
                 continue
@@ -398,11 +409,17 @@ class Server:
                 self.inflight[L] = (owners, *self._launch(L, tasks, small))
                 self.slices += 1
                 progressed = True
+        t1 = time.perf_counter()
         self._admit(now)  # overlaps the slices
+        t2 = time.perf_counter()
         for L in list(self.inflight):
             if self.inflight[L][3].query():
                 self._collect(L)
                 progressed = True
+        t3 = time.perf_counter()
+        self.stats["host_launch_s"] += t1 - t0
+        self.stats["host_admit_s"] += t2 - t1
+        self.stats["host_collect_s"] += t3 - t2
         return progressed
 
     def serve(self, requests: list[Request], t0: float | None = None, drain_s: float = 600.0) -> list[Result]:
