@@ -137,11 +137,25 @@ def cpu_baseline(engine, per_class, cfg, n_tasks: int):
         "cores": int(o.wlo_max_threads()),
         "kind": "port",
         "sample": f"{k} samples x 4 classes ({st.S} tasks) of the same workload run to ln f<=1e-6 by the "
-                  f"oracle's single-threaded transliteration, OpenMP over tasks; {dt:.1f} s, failed={failed}",
+                  f"oracle's single-threaded transliteration, OpenMP over tasks; {dt:.1f} s, failed={failed}; "
+                  f"host {host_cpu()}",
         "attempts_per_s": attempts / dt,
         "seconds": dt,
-        "cpu": platform.processor() or platform.machine(),
+        "cpu": host_cpu(),
     }
+
+
+def host_cpu() -> str:
+    """CPU model and logical core count of this host."""
+    model = platform.processor() or platform.machine()
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{model}, nproc {os.cpu_count()}"
 
 
 def log(msg: str) -> None:
