@@ -1,8 +1,8 @@
 // Checks of the prerun/mainrun file layer (qec_pf_wang_landau_amd/csrc/wl_io.cpp)
 // against the vectors of the reference's own file tests (test/test_file_io.cpp:
-// histogram, interaction and lattice write/read) and its path scheme
-// (test/integration_mainrun.py:91-95), plus the ln g result round trip of
-// src/utils.cpp:137-236.  Built and run by tests/test_file_io.py.
+// histogram, interaction and lattice write/read; test/test_result_handling.cpp:
+// the ln g result dataset and its CodeVersion attribute) and its path scheme
+// (test/integration_mainrun.py:91-95).  Built and run by tests/test_file_io.py.
 //
 //   io_check <scratch directory>
 // prints one line per failed check and exits non-zero if any failed.
@@ -123,6 +123,21 @@ int main(int argc, char** argv)
     }
   }
   CHECK(seen == 3);
+
+  // --- the reference's result-writing vector (test/test_result_handling.cpp:43-105):
+  // two interval maps flattened in order, disorder 100, seed 42
+  const std::string res2 = root + "/results2";
+  wlio::create_directories(res2);
+  wlio::write_results({1, 2, 3, 4}, {0.5, 1.5, 2.5, 3.5}, 100, 42, res2, "timestamp_01:01:2024",
+                      "release_0001-10-h76if76a-dirty");
+  const std::vector<wlio::result_record> r2 = wlio::read_results(res2 + "/mainrun_results.h5");
+  CHECK(r2.size() == 1);
+  if (r2.size() == 1) {
+    CHECK(r2[0].disorder_id == 100 && r2[0].seed == 42 && r2[0].timestamp == "timestamp_01:01:2024");
+    CHECK((r2[0].E == std::vector<int>{1, 2, 3, 4}));
+    CHECK((r2[0].lng == std::vector<double>{0.5, 1.5, 2.5, 3.5}));
+    CHECK(r2[0].version == "release_0001-10-h76if76a-dirty");
+  }
 
   // --- paths (test/integration_mainrun.py:91-95)
   wlio::prerun_options po;
