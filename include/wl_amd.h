@@ -296,6 +296,12 @@ size_t wl_block_interval_lds_bytes(const wl_task* task, const int* h_start, cons
  * reference. */
 int wl_exact_dos(const int8_t* d_J, int X, int Y, int boundary, int num_samples, uint64_t* d_counts, void* stream);
 
+/* Test support: runs the checks of the reference's test/test_device_ptr.cu
+ * on the owning device buffer the host drivers use (the counterpart of
+ * wl::device_ptr, src/cuda_utils.cuh:112-265) and sets one bit of
+ * *failed_mask per failed check (0 = all passed). */
+int wl_selftest_device_buffer(int* failed_mask);
+
 /* LDS read-bandwidth probe: `blocks` workgroups of 256 threads stream
  * reps x 8 reads of 16 bytes each from LDS (ds_read_b128) and write one word
  * per thread to d_sink (blocks x 256 u32).  wl_probe_lds_read_bytes gives the

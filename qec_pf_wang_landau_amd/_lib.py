@@ -52,6 +52,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "wl_block_interval_lds_bytes": (SZ, [P, P, P]),
     "wl_block_run": (I32, [P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, I32, I32, P, P]),
     "wl_exact_dos": (I32, [P, I32, I32, I32, I32, P, P]),
+    "wl_selftest_device_buffer": (I32, [P]),
     "wl_probe_lds_read": (I32, [I32, I32, P, P]),
     "wl_probe_lds_read_bytes": (F64, [I32, I32]),
     "wl_stitch_work_bytes": (SZ, [I32, I32]),

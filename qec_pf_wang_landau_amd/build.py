@@ -26,7 +26,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.environ.get("HIPCC", f"{ROCM}/bin/hipcc")
 ARCH = os.environ.get("WL_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["wl_plain.hip", "wl_fused.hip", "wl_block.hip", "wl_decode.hip", "wl_exact.hip", "wl_probe.hip"]
+HIP_SOURCES = ["wl_plain.hip", "wl_fused.hip", "wl_block.hip", "wl_decode.hip", "wl_exact.hip", "wl_probe.hip",
+               "wl_selftest.hip"]
 HOST_SOURCES = ["wl_host.cpp"]
 EXECUTABLES = {"prerun": "prerun_main.cpp", "mainrun": "mainrun_main.cpp", "results_dump": "results_dump.cpp",
                "decode_results": "decode_results.cpp"}

@@ -101,3 +101,13 @@ def test_lds_read_probe_measures_a_plausible_peak(cuda):
     assert 1e12 < bw < 1e15
     with pytest.raises(WLError):
         call("wl_probe_lds_read", 0, 10, None, None)
+
+
+def test_device_buffer_checks_of_the_reference(cuda):
+    """The owning device buffer of the host drivers passes the checks of the
+    reference's test/test_device_ptr.cu (sizes, host-range initialisation, a
+    kernel write, every out-of-range copy refused)."""
+    import ctypes as C  # noqa: PLC0415
+    mask = C.c_int(-1)
+    call("wl_selftest_device_buffer", C.byref(mask))
+    assert mask.value == 0, f"failed checks: {mask.value:#x}"
