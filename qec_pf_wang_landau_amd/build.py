@@ -30,7 +30,7 @@ HIP_SOURCES = ["wl_plain.hip", "wl_fused.hip", "wl_block.hip", "wl_decode.hip", 
                "wl_selftest.hip"]
 HOST_SOURCES = ["wl_host.cpp"]
 EXECUTABLES = {"prerun": "prerun_main.cpp", "mainrun": "mainrun_main.cpp", "results_dump": "results_dump.cpp",
-               "decode_results": "decode_results.cpp"}
+               "decode_results": "decode_results.cpp", "prerun_dump": "prerun_dump.cpp"}
 CLI_SOURCES = ["wl_io.cpp"]
 
 COMMON = [

@@ -76,3 +76,69 @@ def test_prerun_mainrun_cli_match_library(cuda, tmp_path):
         E, lng = engine.stitch(got.sample_logG(s), a, b, cfg.walker_per_interval, cfg.X, cfg.Y)
         assert by[s][0]["E"] == E.tolist()
         assert np.array_equal(np.array(by[s][0]["lng"]), lng)
+
+
+def prerun_dump(directory: Path, X: int, Y: int) -> dict:
+    from test_file_io import parse_prerun_dump  # noqa: PLC0415
+    return parse_prerun_dump(run([str(BIN / "prerun_dump"), str(directory), str(X), str(Y)], directory))
+
+
+def all_energies(J: np.ndarray, X: int, Y: int, boundary: int) -> set:
+    """Every energy of the X x Y lattice under couplings J (brute force)."""
+    N = X * Y
+    codes = np.arange(1 << N, dtype=np.int64)
+    spins = (1 - 2 * ((codes[:, None] >> np.arange(N)) & 1)).astype(np.int8)
+    return set(engine.lattice_energies(spins, J, X, Y, boundary).tolist())
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_prerun_files_pass_the_reference_integration_checks(cuda, tmp_path, boundary):
+    """The checks of the reference's prerun integration scripts on this
+    prerun's HDF5 output: every stored seed lattice has the energy it is
+    filed under (test/integration_prerun_check_energies.py), and every
+    sample's histogram flags are exactly the energies of all 2^16 lattices
+    (test/integration_prerun_matching_hist_energies.py: 4x4, 10 samples,
+    10000 steps, 128 walkers, 20 intervals, p in {0, 1})."""
+    if not (BIN / "prerun_dump").exists():
+        pytest.skip("executables built without HDF5")
+    X = Y = 4
+    S = 10
+    for prob in (0.0, 1.0):
+        cwd = tmp_path / f"p{prob}"
+        cwd.mkdir()
+        run([str(BIN / "prerun"), "-x", str(X), "-y", str(Y), "-p", str(prob), "-n", "10000", "-w", "128", "-s", "99",
+             "-i", "20", "-r", str(S), "-d", "99", "--boundary", str(boundary)], cwd)
+        files = list(cwd.rglob("prerun_results.h5"))
+        assert len(files) == 1
+        dump = prerun_dump(files[0].parent, X, Y)
+        assert sorted(dump["histogram"]) == list(range(S))
+        for s in range(S):
+            J = np.array(dump["interaction"][s], np.int8)
+            lats = {E: lat for (ss, E), lat in dump["lattice"].items() if ss == s}
+            assert lats, f"sample {s}: no seed lattice stored"
+            for E, lat in lats.items():
+                got = int(engine.lattice_energies(np.array(lat, np.int8)[None, :], J, X, Y, boundary)[0])
+                assert got == E, (s, E, got)
+            flags = np.array(dump["histogram"][s])
+            found = set((np.flatnonzero(flags) - 2 * X * Y).tolist())
+            assert found == all_energies(J, X, Y, boundary), s
+
+
+def test_prerun_coupling_fraction_matches_p(cuda, tmp_path):
+    """test/integration_prerun_check_interactions.py: the share of flipped
+    couplings of the stored disorder is p within 0.02 (16x16, 20 samples)."""
+    if not (BIN / "prerun_dump").exists():
+        pytest.skip("executables built without HDF5")
+    X = Y = 16
+    for boundary in (0, 1):
+        for prob in (0.07, 0.31):
+            cwd = tmp_path / f"b{boundary}_{prob}"
+            cwd.mkdir()
+            run([str(BIN / "prerun"), "-x", str(X), "-y", str(Y), "-p", str(prob), "-n", "2000", "-w", "64", "-s",
+                 "7", "-i", "16", "-r", "20", "-d", "1", "--boundary", str(boundary)], cwd)
+            files = list(cwd.rglob("prerun_results.h5"))
+            assert len(files) == 1
+            dump = prerun_dump(files[0].parent, X, Y)
+            J = np.concatenate([np.array(v) for v in dump["interaction"].values()])
+            assert J.size == 20 * 2 * X * Y
+            assert abs(np.mean(J == -1) - prob) < 0.02
