@@ -26,7 +26,7 @@ def run(args, cwd):
     return p.stdout
 
 
-def dump(path: Path):
+def dump_results(path: Path):
     recs = []
     for line in run([str(BIN / "results_dump"), str(path)], path.parent).splitlines():
         if line.startswith("#"):
@@ -52,7 +52,7 @@ def test_prerun_mainrun_cli_match_library(cuda, tmp_path):
     run([str(BIN / "mainrun"), *common, *main, "--batched"], tmp_path)
     res = list(tmp_path.rglob("mainrun_results.h5"))
     assert len(res) == 1
-    recs = dump(res[0])
+    recs = dump_results(res[0])
     # two samples x two runs (plain, batched), in two timestamp groups
     assert len(recs) == 4
     by = {}
@@ -142,3 +142,33 @@ def test_prerun_coupling_fraction_matches_p(cuda, tmp_path):
             J = np.concatenate([np.array(v) for v in dump["interaction"].values()])
             assert J.size == 20 * 2 * X * Y
             assert abs(np.mean(J == -1) - prob) < 0.02
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_mainrun_files_pass_the_reference_integration_check(cuda, tmp_path, boundary):
+    """test/integration_mainrun.py on this CLI's output (5x5, p = 0.2, 10
+    samples; prerun 10000 steps, 128 walkers, 36 intervals; mainrun 1000
+    steps, 10 intervals x 4 walkers, alpha 0.8, beta 0.001, overlap 0.25,
+    exchange every 20): for every sample, the stored ln g covers exactly the
+    prerun's energies and its log-sum-exp is N ln 2."""
+    if not (BIN / "prerun_dump").exists():
+        pytest.skip("executables built without HDF5")
+    X = Y = 5
+    S = 10
+    common = ["-x", str(X), "-y", str(Y), "-p", "0.2", "-s", "1234", "-e", "h", "-r", str(S), "-d", "99",
+              "--boundary", str(boundary)]
+    run([str(BIN / "prerun"), *common, "-n", "10000", "-w", "128", "-i", "36"], tmp_path)
+    run([str(BIN / "mainrun"), *common, "-n", "1000", "-a", "0.8", "-b", "0.001", "-i", "10", "-w", "4", "-o", "0.25",
+         "-c", "20", "-f", "3600", "--batched"], tmp_path)
+    pre = list(tmp_path.rglob("prerun_results.h5"))
+    res = list(tmp_path.rglob("mainrun_results.h5"))
+    assert len(pre) == 1 and len(res) == 1
+    dump = prerun_dump(pre[0].parent, X, Y)
+    recs = dump_results(res[0])
+    assert sorted(r["disorder"] for r in recs) == list(range(S))
+    for r in recs:
+        flags = np.array(dump["histogram"][r["disorder"]])
+        assert set(r["E"]) == set((np.flatnonzero(flags) - 2 * X * Y).tolist())
+        g = np.array(r["lng"])
+        m = g.max()
+        assert m + np.log(np.exp(g - m).sum()) == pytest.approx(X * Y * np.log(2.0), abs=1e-10)
