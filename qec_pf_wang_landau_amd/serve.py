@@ -64,7 +64,8 @@ class Result:
     arrival: float
     admitted: float
     done: float
-    status: int = 1         # 1 = decoded, otherwise the first failing task status
+    status: int = 1         # 1 = decoded, otherwise the first failing task status (-1: set-up failed)
+    error: str = ""         # why the set-up failed, when it did
 
     @property
     def latency(self) -> float:
@@ -229,7 +230,15 @@ class Server:
             if not fut.done():
                 still.append(item)
                 continue
-            st = fut.result()
+            try:
+                st = fut.result()
+            except Exception as exc:  # noqa: BLE001 - a bad request must not stop the service
+                for k, o in offs.items():
+                    self.arena[k].release(o, need[k])
+                now = time.perf_counter()
+                self.results.append(Result(req.rid, req.L, np.full(4, np.nan), -1, [], [], np.zeros(4, np.int64),
+                                           req.arrival, admitted, now, -1, f"{type(exc).__name__}: {exc}"))
+                continue
             sizes = {"lattice": st.lattice.size, "J": st.J.size, "hist": st.logG.size, "walker": st.E.size,
                      "interval": st.start.size, "spectrum": st.spectrum.size}
             if any(sizes[k] > need[k] for k in sizes):

@@ -74,3 +74,34 @@ def test_decisions_prefer_the_error_class_at_low_p(cuda):
     assert np.mean([r.decision == 0 for r in res]) >= 0.75
     s = serve.latency_summary(res)
     assert s["decoded"] == 4 and s["p99_s"] >= s["p50_s"] > 0
+
+
+class _FailingSetup(serve.Server):
+    """A server whose set-up of one request raises (forced from the host)."""
+
+    def __init__(self, bad_rid, **kw):
+        super().__init__(**kw)
+        self.bad_rid = bad_rid
+
+    def _setup(self, req):
+        if req.rid == self.bad_rid:
+            raise RuntimeError("set-up refused for this test")
+        return super()._setup(req)
+
+
+def test_failed_setup_is_reported_and_the_service_continues(cuda):
+    """A request whose set-up fails comes back with status -1 and the error,
+    its memory is released, and the other requests decode exactly as on a
+    server without the failure."""
+    reqs = [serve.Request(20, 6, 0.05, seed=801), serve.Request(21, 6, 0.05, seed=802),
+            serve.Request(22, 4, 0.05, seed=803)]
+    srv = _FailingSetup(21, hbm_bytes=64 << 20, slice_rounds=100, sizes=SIZES)
+    got = by_rid(srv.serve([serve.Request(r.rid, r.L, r.p, r.seed, 0.0) for r in reqs]))
+    assert got[21].status == -1 and "set-up refused" in got[21].error and got[21].decision == -1
+    ref = by_rid(serve.Server(hbm_bytes=64 << 20, slice_rounds=100, sizes=SIZES).serve(
+        [serve.Request(r.rid, r.L, r.p, r.seed, 0.0) for r in reqs if r.rid != 21]))
+    for rid in (20, 22):
+        same(got[rid], ref[rid])
+    for a in srv.arena.values():
+        assert a.used == 0
+    assert serve.latency_summary(list(got.values()))["failed"] == 1
