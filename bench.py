@@ -59,6 +59,7 @@ def parse():
     ap.add_argument("--curve-L", type=int, nargs="+", default=[4, 6], help="curve: lattice sizes")
     ap.add_argument("--curve-p", type=float, nargs="+", default=[0.04, 0.07, 0.10, 0.13],
                     help="curve: flip probabilities")
+    ap.add_argument("--curve-no-exact", action="store_true", help="curve: skip the exact-enumeration comparison")
     ap.add_argument("--prerun-iters", type=int, default=1000000, help="prerun steps for 16x16 and 32x32")
     ap.add_argument("--prob", type=float, default=0.1, help="32x32: flip probability (0 = the clean torus)")
     ap.add_argument("--converge", action="store_true",
@@ -515,16 +516,18 @@ def bench_curve(args, engine, torch) -> dict:
     Ls, ps = args.curve_L, args.curve_p
     error_rate.decode_point(Ls[0], ps[0], 4, error_rate.run_config(Ls[0], seed=1))  # warmup
     points, total_s, decoded, attempts = [], 0.0, 0, 0
+    curves = {L: [] for L in Ls}
     for L in Ls:
         cfg = error_rate.run_config(L, seed=12345)
         for prob in ps:
             wl = error_rate.decode_point(L, prob, S, cfg)
+            curves[L].append(wl)
             total_s += wl.seconds
             decoded += wl.decoded
             attempts += wl.attempts
             row = {"L": L, "p": prob, "wl": {"rate": wl.rate, "wilson": [wl.lo, wl.hi], "decoded": wl.decoded,
                                              "seconds": wl.seconds, "rounds_max": wl.extra.get("rounds_max")}}
-            if L * L <= 40:
+            if L * L <= 40 and not args.curve_no_exact:
                 ex = error_rate.decode_point(L, prob, S, cfg, method="exact")
                 srt = np.sort(ex.log_z, axis=1)
                 clear = (srt[:, -1] - srt[:, -2]) > 0.5
@@ -538,6 +541,8 @@ def bench_curve(args, engine, torch) -> dict:
             log(f"curve L={L} p={prob}: wl rate {wl.rate:.4f} [{wl.lo:.4f}, {wl.hi:.4f}] in {wl.seconds:.1f}s"
                 + (f"; exact {row['exact']['rate']:.4f}, agree {row['decisions_agree']:.3f}" if "exact" in row
                    else ""))
+    crossings = {f"{a}-{b}": error_rate.crossing(curves[a], curves[b]) for a, b in zip(Ls, Ls[1:])}
+    log(f"curve crossings (p where the rates of two sizes cross): {crossings}")
     achieved = attempts * BYTES_PER_ATTEMPT / total_s / 1e9
     return {
         "metric": METRIC, "value": decoded / total_s, "unit": "samples/s", "n_gpus": 1, "steps": len(points),
@@ -548,6 +553,7 @@ def bench_curve(args, engine, torch) -> dict:
                                "timed per point: couplings, prerun, Wang-Landau, device stitching and ln Z",
                    "samples_per_point": S},
         "points": points,
+        "crossings": crossings,
         "attempts_per_s": attempts / total_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
