@@ -21,14 +21,15 @@
 //
 // Per interval in LDS:
 //   JJ[X]          coupling signs as bits, {right, down} per lattice row
-//                  (Y <= 32)
+//                  (Y <= 32; right bonds held twice for Y <= 16)
 //   mask[R]        the task's spectrum flags over [E_min, E_max], and the
 //                  same flags as a bit array offset by 8 (wl_sweep.h)
 //   G[nb+2P][W]    ln g per walker over the interval's reachable energies
 //                  e0 + stride*b, walker-minor, with P = 8 >> shift zero guard
 //                  rows on both sides
 //   Hc[nb][W]      histogram, u32
-//   lat[W][X]      the walkers' lattices as bits
+//   lat[W][X]      the walkers' lattices as bits (rows held twice for
+//                  Y <= 16)
 //   EL[W]          energies
 // Same Philox streams, draw counts and FP64 operation order as the plain
 // kernels and the one-workgroup-per-task kernel: results are bit-identical.
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
       h |= static_cast<uint32_t>(Jg[r * Y + c] < 0) << c;
       v |= static_cast<uint32_t>(Jg[N + r * Y + c] < 0) << c;
     }
-    JJ[r] = make_uint2(h, v);
+    JJ[r] = make_uint2(row_store(h, Y), v);
   }
   for (int e = tid; e < R; e += nthr) {
     mask[e] = st.spectrum[t.spectrum_off + e];
@@ -285,7 +286,7 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
           for (int c = 0; c < Y; ++c) {
             b |= static_cast<uint32_t>(src[r * Y + c] < 0) << c;
           }
-          lat[tid * X + r] = b;
+          lat[tid * X + r] = row_store(b, Y);
         }
       }
       const int e = EL[tid];
@@ -358,9 +359,17 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
       ws.g = G[ws.b * W + tid];
       const sweep_ctx cx{Lw, JJ, mbits, G + tid, Hc + tid, X, Y, emin, sk, ek, shift, W, upd, lnf};
       if (t.boundary == 1) {
-        sweep<1>(cx, ws, rng, att, t.num_iterations);
+        if (rows_doubled(Y)) {
+          sweep<1, true>(cx, ws, rng, att, t.num_iterations);
+        } else {
+          sweep<1, false>(cx, ws, rng, att, t.num_iterations);
+        }
       } else {
-        sweep<0>(cx, ws, rng, att, t.num_iterations);
+        if (rows_doubled(Y)) {
+          sweep<0, true>(cx, ws, rng, att, t.num_iterations);
+        } else {
+          sweep<0, false>(cx, ws, rng, att, t.num_iterations);
+        }
       }
       if (!ws.alive) {
         misc[0] = 1;

@@ -8,7 +8,8 @@
 //
 // LDS layout of a task (all counts from the task descriptor):
 //   JJ[X]                 coupling signs as bits (1 = antiferromagnetic),
-//                         {right, down} per lattice row (Y <= 32)
+//                         {right, down} per lattice row (Y <= 32; the right
+//                         bonds held twice for Y <= 16, see row_store)
 //   mask[R], mbits        spectrum flags over [E_min, E_max], bytes and the
 //                         bit array of wl_sweep.h
 //   per interval          start, end, first reachable energy e0, compact bin
@@ -20,6 +21,7 @@
 //                         zero guard rows on both sides
 //   Hc[rows][W]           histogram, u32, same indexing
 //   lat[M][X]             lattices as bits (1 = spin -1), one u32 per row
+//                         (held twice for Y <= 16, see row_store)
 //   EL[M], slot[M], idx[M] energy, lattice slot and replica index per walker
 // Lattices never move: replica exchange swaps slots and energies, as the
 // reference swaps lattice offsets (src/wl.cu:880-885).
@@ -239,7 +241,7 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       h |= static_cast<uint32_t>(Jg[r * Y + c] < 0) << c;
       v |= static_cast<uint32_t>(Jg[N + r * Y + c] < 0) << c;
     }
-    JJ[r] = make_uint2(h, v);
+    JJ[r] = make_uint2(row_store(h, Y), v);
   }
   for (int e = tid; e < p.R; e += nthr) {
     mask[e] = st.spectrum[t.spectrum_off + e];
@@ -283,7 +285,7 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       for (int c = 0; c < Y; ++c) {
         b |= static_cast<uint32_t>(src[r * Y + c] < 0) << c;
       }
-      lat[l * X + r] = b;
+      lat[l * X + r] = row_store(b, Y);
     }
     EL[l] = st.E[wbase + l];
     SL[l] = static_cast<int>((st.off_lattice[wbase + l] - t.lattice_off) / N);
@@ -363,9 +365,17 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       ws.g = Gw[ws.b * W];
       const sweep_ctx cx{L, JJ, mbits, Gw, Hw, X, Y, emin, sk, ek, shift, W, upd, lnf};
       if (t.boundary == 1) {
-        sweep<1>(cx, ws, rng, att, t.num_iterations);
+        if (rows_doubled(Y)) {
+          sweep<1, true>(cx, ws, rng, att, t.num_iterations);
+        } else {
+          sweep<1, false>(cx, ws, rng, att, t.num_iterations);
+        }
       } else {
-        sweep<0>(cx, ws, rng, att, t.num_iterations);
+        if (rows_doubled(Y)) {
+          sweep<0, true>(cx, ws, rng, att, t.num_iterations);
+        } else {
+          sweep<0, false>(cx, ws, rng, att, t.num_iterations);
+        }
       }
       if (!ws.alive) {
         misc[0] = 1;

@@ -354,14 +354,15 @@ WL_HD bool accept_exact(double d, uint32_t x)
 #if defined(__HIPCC__) || defined(__HIP__)
 // Same decision as accept_exact, faster on the device, in the log domain:
 // u < exp(d) iff ln u < d.  ln u is estimated in FP32 from the word alone
-// (hardware log2 of the FP32 u, times ln 2), so it can be computed before d
-// is known; d is compared against the bounds ln u +- 2^-14.  Error of the
-// estimate for u in [2^-33, 1): FP32 u within 2^-23 relative (ln error <
-// 2^-22), log2 within 2 ulp of a result of magnitude < 33 (< 2^-17), the
-// ln 2 product 1 ulp (< 2^-19); d itself is compared in FP64 against the
-// widened bounds.  The sum is < 2^-15.5, so outside the band both
-// comparisons agree; d >= 0 accepts (u < 1); only the band case, d within
-// 2^-14 of ln u, falls back to the FP64 rule and branches.
+// (hardware log2 of the FP32 u, times ln 2 in one fused multiply-add with
+// the band offset), so it can be computed before d is known; d is compared
+// against the bounds ln u +- 2^-14.  Error of the estimate for u in
+// [2^-33, 1): FP32 u within 2^-23 relative (ln error < 2^-22), log2 within
+// 2 ulp of a result of magnitude < 33 (< 2^-17.4 after the ln 2 factor),
+// the FP32 ln 2 constant (< 2^-20.8 at |log2 u| < 33) and the rounding of
+// the fused result (< 2^-20).  The sum is < 2^-16.5, so outside the band
+// both comparisons agree; d >= 0 accepts (u < 1); only the band case, d
+// within 2^-14 of ln u, falls back to the FP64 rule and branches.
 struct accept_band {
   float lo, hi;
 };
@@ -370,35 +371,38 @@ __device__ __forceinline__ accept_band accept_bounds(uint32_t x)
 {
   const float u = __builtin_fmaf(static_cast<float>(x), 0x1p-32f, 0x1p-33f);
 #if defined(__HIP_DEVICE_COMPILE__)
-  const float lu = __builtin_amdgcn_logf(u) * 0x1.62e430p-1f;
+  const float l2u = __builtin_amdgcn_logf(u);
 #else
-  const float lu = 0.0f * u;  // the host pass never executes device code
+  const float l2u = 0.0f * u;  // the host pass never executes device code
 #endif
-  return accept_band{lu - 0x1p-14f, lu + 0x1p-14f};
+  return accept_band{__builtin_fmaf(l2u, 0x1.62e430p-1f, -0x1p-14f), __builtin_fmaf(l2u, 0x1.62e430p-1f, 0x1p-14f)};
 }
 
-// The band as two FP64 limits, computed off the chain: accept iff d > t,
+// The band as two FP32 limits, computed off the chain: accept iff d > t,
 // and d in [lo, t] takes the exact rule.  d >= 0 or d > hi accepts, so
-// t = hi when hi < 0 and otherwise the largest negative double (d > t is
-// then d >= 0); the rejections d < lo stay rejections.
+// t = hi when hi < 0 and otherwise -2^-126 (d > t then means
+// exp(d) > 1 - 2^-126 > u); the rejections d < lo stay rejections.
 struct accept_limits {
-  double t, lo;
+  float t, lo;
 };
 
 __device__ __forceinline__ accept_limits accept_limits_of(accept_band band)
 {
-  return accept_limits{(band.hi < 0.0f) ? static_cast<double>(band.hi) : -0x1p-1074,
-                       static_cast<double>(band.lo)};
+  return accept_limits{(band.hi < 0.0f) ? band.hi : -0x1p-126f, band.lo};
 }
 
-// The decision from the limits: one FP64 compare on the attempt's chain.  The
-// exact rule sits behind a wave-uniform branch, so the common path changes
+// The decision from the limits: d rounded to FP32 and one compare on the
+// attempt's chain.  Rounding to nearest is monotone and the limits are FP32
+// values, so d32 > t implies d > t and d32 < lo implies d < lo: the fast
+// accept and the fast reject are those of the FP64 d.  The exact rule, on
+// the FP64 d, sits behind a wave-uniform branch, so the common path changes
 // no execution mask.  Limits of +inf reject without the exact rule.
 __device__ __forceinline__ bool accept_limited(double d, uint32_t x, accept_limits lim)
 {
-  bool acc = d > lim.t;
-  const bool inb = (d >= lim.lo) & (d <= lim.t);
-  if (__builtin_amdgcn_ballot_w64(inb) != 0) {
+  const float d32 = static_cast<float>(d);
+  bool acc = d32 > lim.t;
+  const bool inb = (d32 >= lim.lo) & !acc;  // d is never NaN: !(d32 > t) is d32 <= t
+  if (__builtin_expect(__builtin_amdgcn_ballot_w64(inb) != 0, 0)) {
     const bool ex = u32_to_unit(x) < wl_exp(d);
     acc = inb ? ex : acc;
   }

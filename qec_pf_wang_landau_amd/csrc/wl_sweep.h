@@ -4,9 +4,10 @@
 //
 // LDS inputs of a sweep:
 //   L[X]       the walker's spins as bit rows (one u32 per row, Y <= 32; a set
-//              bit is spin -1)
+//              bit is spin -1; for Y <= 16 each row is held twice, row_store)
 //   JJ[X]      per row {right-bond signs, down-bond signs} as bits (a set bit
-//              is an antiferromagnetic bond), 8-byte aligned
+//              is an antiferromagnetic bond; right bonds held twice for
+//              Y <= 16), 8-byte aligned
 //   mbits[]    spectrum flags as a bit array: bit (E - E_min + 8) is set iff E
 //              is a spectrum energy; the 8 bits below E_min and every bit
 //              above E_max are zero, and one extra word follows the last
@@ -18,10 +19,10 @@
 // One attempt issues every LDS read it needs at once: the three lattice
 // rows and two coupling rows around the site, the two spectrum words around
 // the current energy and ln g at the four bins a flip can reach (the energy
-// change of a flip is 2*s*sum over c neighbours, so it is one of +-2c,
-// +-(2c-4) or 0, whatever the spin).  The chain of an attempt is therefore
-// one LDS round trip long; the Philox block of the next pair of attempts is
-// computed in the shadow of those reads.
+// change of a flip is 2 (c - 2u) for c neighbours of which u are joined by
+// unsatisfied bonds, so it is one of +-2c, +-(2c-4) or 0).  The chain of an
+// attempt is therefore one LDS round trip long; the Philox block of the
+// next pair of attempts is computed in the shadow of those reads.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -59,6 +60,12 @@ struct sweep_ctx {
   bool upd;               // interval still active: ln g and H are updated
   double lnf;
 };
+
+// a ^ b ^ c in one three-input bit operation (truth table 0x96)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
 
 // Philox4x32-10 of wl_math.h (counter (block lo, block hi, slot, 0), key
 // (seed + sample, walk tag)), computed in three parts: the first two rounds
@@ -106,14 +113,35 @@ struct philox_run {
   }
 };
 
+// Rows held twice: for Y <= 16 every lattice row and every right-bond row
+// keeps its Y bits at [0, Y) and again at [Y, 2Y), so the left neighbour of
+// column j is bit j + Y - 1 and the right neighbour bit j + 1, with no wrap
+// arithmetic.  Wider rows (Y <= 32) are held once.
+__host__ __device__ __forceinline__ bool rows_doubled(int Y) { return Y <= 16; }
+
+__host__ __device__ __forceinline__ uint32_t row_store(uint32_t bits, int Y)
+{
+  return rows_doubled(Y) ? (bits | (bits << Y)) : bits;
+}
+
+// a * b for |a|, |b| < 2^23 on the full-rate 24-bit multiplier (the
+// compiler cannot always prove the operand ranges for __mul24 itself)
+__device__ __forceinline__ int mul_i24(int a, int b)
+{
+  int r;
+  __asm__("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // Everything an attempt reads from LDS, issued before any of it is used.
 struct attempt_reads {
-  int i, j, jnn, jpp, u1, u2;
+  int i, j, jl, u1, u2;
+  uint32_t fm;  // flip mask of the site's row
   uint32_t rc, ru, rd, jh, jvd, jvu, m0, m1;
   double g1, g2, g3, g4;
 };
 
-template<int BOUNDARY>
+template<int BOUNDARY, bool DBL>
 __device__ __forceinline__ attempt_reads attempt_issue(const sweep_ctx& c, const sweep_state& s, uint32_t xs)
 {
   attempt_reads r;
@@ -125,8 +153,10 @@ __device__ __forceinline__ attempt_reads attempt_issue(const sweep_ctx& c, const
   r.j = static_cast<int>((static_cast<uint64_t>(static_cast<uint32_t>(px)) * static_cast<uint32_t>(c.Y)) >> 32);
   const int ipp = (r.i + 1 < c.X) ? r.i + 1 : 0;
   const int inn = (r.i > 0) ? r.i - 1 : c.X - 1;
-  r.jpp = (r.j + 1 < c.Y) ? r.j + 1 : 0;
-  r.jnn = (r.j > 0) ? r.j - 1 : c.Y - 1;
+  // bit of the bond between column j and its left neighbour in the
+  // bond-mismatch word of attempt_finish
+  r.jl = DBL ? r.j + c.Y - 1 : ((r.j > 0) ? r.j - 1 : c.Y - 1);
+  r.fm = DBL ? ((1u << r.j) | (1u << (r.j + c.Y))) : (1u << r.j);
   // reachable energy changes: +-2nc and +-(2nc - 4) for nc neighbours
   int nc = 4;
   if (BOUNDARY == 1) {
@@ -158,26 +188,37 @@ __device__ __forceinline__ attempt_reads attempt_issue(const sweep_ctx& c, const
 // have landed: branch-free except for the rare exact acceptance fallback;
 // every effect is a select or an LDS atomic whose operand is zero when the
 // effect does not apply.
-template<int BOUNDARY>
+template<int BOUNDARY, bool DBL>
 __device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& s, const attempt_reads& r, uint32_t xa,
                                                accept_band band)
 {
   const int j = r.j;
-  const uint32_t sbit = (r.rc >> j) & 1u;
-  // mismatch (s_nb J_nb = -1) per neighbour
-  const uint32_t mu = ((r.ru ^ r.jvu) >> j) & 1u;
-  const uint32_t md = ((r.rd ^ r.jvd) >> j) & 1u;
-  const uint32_t ml = ((r.rc >> r.jnn) ^ (r.jh >> r.jnn)) & 1u;
-  const uint32_t mr = ((r.rc >> r.jpp) ^ (r.jh >> j)) & 1u;
-  int sum;
+  // A bond is unsatisfied (s s' J = -1) iff an odd number of its three
+  // bits (spin, spin, coupling; a set bit is -1) is set.  Bit k of hx is
+  // the bond between columns k and k + 1 of the site's row; bit j of vu and
+  // vd the bonds to the rows above and below.
+  uint32_t hx;
+  if (DBL) {
+    hx = xor3(r.rc, r.rc >> 1, r.jh);
+  } else {
+    // right neighbour of the last column is column 0
+    const uint32_t right = (r.rc >> 1) | (((r.rc & 1u) << (c.Y - 1)));
+    hx = xor3(r.rc, right, r.jh);
+  }
+  const uint32_t vu = xor3(r.rc, r.ru, r.jvu);
+  const uint32_t vd = xor3(r.rc, r.rd, r.jvd);
+  const int ml = static_cast<int>((hx >> r.jl) & 1u);
+  const int mr = static_cast<int>((hx >> j) & 1u);
+  const int mu = static_cast<int>((vu >> j) & 1u);
+  const int md = static_cast<int>((vd >> j) & 1u);
+  // energy change of the flip: 2 sum over bonds of s s' J = 2 (nc - 2 unsatisfied)
+  int de;
   if (BOUNDARY == 1) {
     const int cu = r.i > 0, cd = r.i < c.X - 1, cl = j > 0, cr = j < c.Y - 1;
-    sum = cu + cd + cl + cr - 2 * static_cast<int>(cu * mu + cd * md + cl * ml + cr * mr);
+    de = r.u1 - 4 * ((cu & mu) + (cd & md) + (cl & ml) + (cr & mr));
   } else {
-    sum = 4 - 2 * static_cast<int>(mu + md + ml + mr);
+    de = 8 - 4 * (mu + md + ml + mr);
   }
-  const int neg = -static_cast<int>(sbit);   // 0 or -1
-  const int de = ((2 * sum) ^ neg) - neg;     // 2 s sum, s = +1/-1
   const int en = s.e + de;
   // flags of E - 8 .. E + 23 from the two words around the current energy
   const int rel = s.e - c.emin;
@@ -197,17 +238,17 @@ __device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& 
   const double ghi = (de == r.u1) ? g1 : g2;   // de > 0
   double g_new = (de < 0) ? glo : ghi;
   g_new = (de == 0) ? s.g : g_new;
-  const int bw_new = s.bw + __mul24(de >> c.shift, c.W);  // off the decision's chain
+  const int bw_new = s.bw + mul_i24(de >> c.shift, c.W);  // off the decision's chain
   const bool valid = s.alive & flag;
   // A move that is not valid or leaves the interval gets limits of +inf
-  // (never accepted, never the exact rule), so the decision is the one FP64
+  // (never accepted, never the exact rule), so the decision is the one
   // compare of accept_limited, with no mask logic after it.
   accept_limits lim = accept_limits_of(band);
   const bool ok = valid & in_range;
-  lim.t = ok ? lim.t : __builtin_inf();
-  lim.lo = ok ? lim.lo : __builtin_inf();
+  lim.t = ok ? lim.t : __builtin_inff();
+  lim.lo = ok ? lim.lo : __builtin_inff();
   const bool acc = accept_limited(s.g - g_new, xa, lim);
-  c.L[r.i] = r.rc ^ (acc ? (1u << j) : 0u);  // the row is this walker's alone: no atomic needed
+  c.L[r.i] = r.rc ^ (acc ? r.fm : 0u);  // the row is this walker's alone: no atomic needed
   s.e = acc ? en : s.e;
   s.bw = acc ? bw_new : s.bw;
   s.g = acc ? g_new : s.g;
@@ -224,7 +265,7 @@ __device__ __forceinline__ void attempt_finish(const sweep_ctx& c, sweep_state& 
 // five rounds [rest of attempt 2]; the fences keep each half of the next
 // block between an attempt's reads and their first use.  Returns the
 // attempts that counted: those made while the walker was alive.
-template<int BOUNDARY>
+template<int BOUNDARY, bool DBL>
 __device__ __forceinline__ uint32_t attempt_pair(const sweep_ctx& cx, sweep_state& ws, const philox4& cur,
                                                  philox_run& nx)
 {
@@ -233,17 +274,17 @@ __device__ __forceinline__ uint32_t attempt_pair(const sweep_ctx& cx, sweep_stat
   const accept_band b1 = accept_bounds(cur.v[1]);
   const accept_band b2 = accept_bounds(cur.v[3]);
   const uint32_t a1 = ws.alive ? 1u : 0u;
-  const attempt_reads r1 = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
+  const attempt_reads r1 = attempt_issue<BOUNDARY, DBL>(cx, ws, cur.v[0]);
   nx.fence();
   nx.rounds<4>();
   nx.fence();
-  attempt_finish<BOUNDARY>(cx, ws, r1, cur.v[1], b1);
+  attempt_finish<BOUNDARY, DBL>(cx, ws, r1, cur.v[1], b1);
   const uint32_t a2 = ws.alive ? 1u : 0u;  // the attempt that meets a new energy counts, later ones do not
-  const attempt_reads r2 = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
+  const attempt_reads r2 = attempt_issue<BOUNDARY, DBL>(cx, ws, cur.v[2]);
   nx.fence();
   nx.rounds<4>();
   nx.fence();
-  attempt_finish<BOUNDARY>(cx, ws, r2, cur.v[3], b2);  // no effect once dead
+  attempt_finish<BOUNDARY, DBL>(cx, ws, r2, cur.v[3], b2);  // no effect once dead
   return a1 + a2;
 }
 
@@ -257,7 +298,7 @@ __device__ __forceinline__ uint32_t attempt_pair(const sweep_ctx& cx, sweep_stat
 // normally the same on every lane: the loop then runs wave-uniform, with no
 // per-lane exit or counter updates.  Lanes that disagree take the per-lane
 // loop below, which gives the same results.
-template<int BOUNDARY>
+template<int BOUNDARY, bool DBL>
 __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, const attempt_stream& rng, uint64_t& att,
                                       int n)
 {
@@ -279,8 +320,8 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
     philox4 cur = rng.eval(blk);
     if ((att0 & 1u) && left > 0) {  // odd start: second half of the block
       done += ws.alive ? 1u : 0u;
-      const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
-      attempt_finish<BOUNDARY>(cx, ws, r, cur.v[3], accept_bounds(cur.v[3]));
+      const attempt_reads r = attempt_issue<BOUNDARY, DBL>(cx, ws, cur.v[2]);
+      attempt_finish<BOUNDARY, DBL>(cx, ws, r, cur.v[3], accept_bounds(cur.v[3]));
       --left;
       ++blk;
       cur = rng.eval(blk);
@@ -288,14 +329,14 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
     for (; left >= 2; left -= 2) {
       philox_run nx;
       nx.begin2(blk + 1, ms_hi, ms_lo, rng.key0);
-      done += attempt_pair<BOUNDARY>(cx, ws, cur, nx);
+      done += attempt_pair<BOUNDARY, DBL>(cx, ws, cur, nx);
       ++blk;
       cur = philox4{{nx.c0, nx.c1, nx.c2, nx.c3}};
     }
     if (left > 0) {
       done += ws.alive ? 1u : 0u;
-      const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
-      attempt_finish<BOUNDARY>(cx, ws, r, cur.v[1], accept_bounds(cur.v[1]));
+      const attempt_reads r = attempt_issue<BOUNDARY, DBL>(cx, ws, cur.v[0]);
+      attempt_finish<BOUNDARY, DBL>(cx, ws, r, cur.v[1], accept_bounds(cur.v[1]));
     }
     att = att0 + done;
     return;
@@ -304,8 +345,8 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
   uint64_t blk = att >> 1;
   philox4 cur = rng.eval(blk);
   if ((att & 1u) && it < n) {
-    const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[2]);
-    attempt_finish<BOUNDARY>(cx, ws, r, cur.v[3], accept_bounds(cur.v[3]));
+    const attempt_reads r = attempt_issue<BOUNDARY, DBL>(cx, ws, cur.v[2]);
+    attempt_finish<BOUNDARY, DBL>(cx, ws, r, cur.v[3], accept_bounds(cur.v[3]));
     ++att;
     ++it;
     ++blk;
@@ -314,13 +355,13 @@ __device__ __forceinline__ void sweep(const sweep_ctx& cx, sweep_state& ws, cons
   for (; it + 2 <= n && ws.alive; it += 2) {
     philox_run nx;
     nx.begin2(blk + 1, ms_hi, ms_lo, rng.key0);
-    att += attempt_pair<BOUNDARY>(cx, ws, cur, nx);
+    att += attempt_pair<BOUNDARY, DBL>(cx, ws, cur, nx);
     ++blk;
     cur = philox4{{nx.c0, nx.c1, nx.c2, nx.c3}};
   }
   if (it < n && ws.alive) {
-    const attempt_reads r = attempt_issue<BOUNDARY>(cx, ws, cur.v[0]);
-    attempt_finish<BOUNDARY>(cx, ws, r, cur.v[1], accept_bounds(cur.v[1]));
+    const attempt_reads r = attempt_issue<BOUNDARY, DBL>(cx, ws, cur.v[0]);
+    attempt_finish<BOUNDARY, DBL>(cx, ws, r, cur.v[1], accept_bounds(cur.v[1]));
     ++att;
   }
 }
