@@ -36,9 +36,12 @@ def test_empty_task_tables(cuda):
 
 
 @pytest.mark.parametrize("boundary", [0, 1])
-def test_widest_rows_both_forms(cuda, boundary):
-    """Y = 32: a lattice row is one full u32 word in LDS."""
-    cfg = engine.RunConfig(X=4, Y=32, boundary=boundary, num_intervals=3, walker_per_interval=4, overlap=0.25,
+@pytest.mark.parametrize("Y", [32, 17, 16])
+def test_widest_rows_both_forms(cuda, boundary, Y):
+    """Row widths at the two LDS row layouts' limits: Y = 32 fills one u32
+    word, Y = 17 is the narrowest row held once (the right neighbour of the
+    last column wraps by a shift), Y = 16 the widest row held twice."""
+    cfg = engine.RunConfig(X=4, Y=Y, boundary=boundary, num_intervals=3, walker_per_interval=4, overlap=0.25,
                            alpha=0.8, beta=1e-6, num_iterations=200, exchange_every=3, seed=12345)
     st = engine.build_state(oracle_samples(cfg, 1, 0.1, "h", prerun_walkers=64, prerun_iters=20000), cfg)
     refs = oracle_per_task(st, 12)
