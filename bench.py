@@ -76,16 +76,17 @@ def parse():
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "k_fused_6x6_p0.04_400tasks_traffic.txt"
+BLOCK_TRAFFIC_FILE = ROOT / "profiles" / "k_interval_32x32_p0.1_64x128_traffic.txt"
 
 
-def measured_traffic():
-    """HBM bytes per launch of the batched kernel from the committed counter
+def measured_traffic(path: Path = TRAFFIC_FILE):
+    """HBM bytes per launch of a batched kernel from its committed counter
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same
     counter's reading for a copy of known size), or None without one."""
-    if not TRAFFIC_FILE.exists():
+    if not path.exists():
         return None
     vals = {}
-    for line in TRAFFIC_FILE.read_text().splitlines():
+    for line in path.read_text().splitlines():
         if ":" in line and not line.startswith("#"):
             k, v = line.split(":", 1)
             vals[k.strip()] = v.strip()
@@ -355,7 +356,14 @@ def bench_32x32(args, engine, torch) -> dict:
     log(f"32x32: {rounds} rounds in {dt:.2f}s, status {res.arrays['status'].tolist()}")
     copy_probe(args, torch)
     cpu = None if args.no_cpu else cpu_rounds_baseline(engine, states[0], cfg, 3)
+    peaks = None if args.no_peaks else measured_peaks(engine, torch)
     achieved = attempts * BYTES_PER_ATTEMPT / dt / 1e9
+    # one launch of the dominant kernel = one block of 20 rounds of k_interval
+    traffic = measured_traffic(BLOCK_TRAFFIC_FILE) if args.prob == 0.1 and S == 1 else None
+    lds = None
+    if peaks is not None:
+        lds = {"achieved": achieved, "peak": peaks["lds_read_TBs"] * 1e3, "unit": "GB/s",
+               "frac": achieved / (peaks["lds_read_TBs"] * 1e3)}
     return {
         "metric": METRIC, "value": attempts / dt, "unit": "attempts/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -366,7 +374,8 @@ def bench_32x32(args, engine, torch) -> dict:
                    "rounds": rounds},
         "ns_per_round": dt / max(rounds, 1) * 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "lds_roofline": lds, "measured_peaks": peaks,
         "cpu_baseline": cpu,
     }
 
