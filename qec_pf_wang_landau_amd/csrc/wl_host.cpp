@@ -200,6 +200,9 @@ int wl_decide_classes(const double* log_z, int num_samples, int num_classes, int
     if (num_samples < 0 || num_classes < 1) {
       throw std::invalid_argument("decide_classes: need num_samples >= 0 and num_classes >= 1");
     }
+    if (num_samples > 0 && (log_z == nullptr || decision == nullptr || fail_weight == nullptr)) {
+      throw std::invalid_argument("decide_classes: log_z, decision and fail_weight are required");
+    }
     for (int s = 0; s < num_samples; ++s) {
       const double* z = log_z + static_cast<int64_t>(s) * num_classes;
       bool finite = true;
@@ -232,6 +235,9 @@ int wl_wilson_interval(double failures, int n, double z, double* lo, double* hi)
   return guarded_host([&] {
     if (n < 1 || failures < 0.0 || failures > static_cast<double>(n) || !(z > 0.0)) {
       throw std::invalid_argument("wilson_interval: need n >= 1, 0 <= failures <= n and z > 0");
+    }
+    if (lo == nullptr || hi == nullptr) {
+      throw std::invalid_argument("wilson_interval: lo and hi are required");
     }
     const double nn = static_cast<double>(n);
     const double z2 = z * z;

@@ -80,6 +80,18 @@ def test_wilson_interval_rejects_bad_input():
             error_rate.wilson(*args)
 
 
+def test_missing_output_pointers_are_refused():
+    """A null output is an argument error (status 1), never a crash."""
+    import ctypes as C  # noqa: PLC0415
+    z = np.zeros((2, 4))
+    dec = np.zeros(2, np.int32)
+    assert _lib.lib().wl_decide_classes(z.ctypes.data_as(C.c_void_p), 2, 4, None,
+                                      dec.ctypes.data_as(C.c_void_p)) == 1
+    assert _lib.lib().wl_decide_classes(None, 0, 4, None, None) == 0
+    lo = C.c_double()
+    assert _lib.lib().wl_wilson_interval(C.c_double(1.0), 10, C.c_double(1.96), C.byref(lo), None) == 1
+
+
 def _pt(L, p, rate):
     return error_rate.Point(L, p, "wl", 10, 10, rate * 10, rate, 0.0, 1.0, 1.96, np.zeros((10, 4)),
                             np.zeros(10, np.int32), np.ones((10, 4), np.int32), 0.0)
