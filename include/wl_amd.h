@@ -17,7 +17,9 @@
  *
  * Every function returns 0 on success, 1 for an invalid argument and 2 for a
  * HIP/runtime failure; wl_last_error() gives the message.  No function
- * throws across this boundary.
+ * throws across this boundary.  A null buffer that a call would read or write
+ * is an invalid argument (buffers documented as optional may be null); a
+ * call with nothing to do (no walkers, samples or tasks) needs no buffers.
  *
  * Types: the reference's `unsigned long` counters are uint64_t, its
  * `unsigned long long` histograms are uint64_t, `signed char`/`int8_t`

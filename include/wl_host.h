@@ -25,7 +25,8 @@ int wl_interval_results(int e_min, int e_max, unsigned int num_intervals, int nu
 int wl_energy_spectrum(const uint8_t* hist, int len_hist, int X, int Y, int* E_min, int* E_max, int8_t* mask,
                        int* len_mask);
 
-/* Spacing of the energies in a spectrum mask (4 periodic, 2 open). */
+/* Spacing of the energies in a spectrum mask (4 periodic, 2 open; 0 for a
+ * single energy or a null mask). */
 int wl_energy_stride(const int8_t* mask, int len);
 
 /* get_lattice_from_prerun_results for one sample, src/file_io.cpp:618-659.
@@ -39,8 +40,9 @@ int wl_select_seed_lattices(const int8_t* found, const int* found_energy, int nu
 int wl_stitch_log_g(const double* logG_sample, const int* start, const int* end, int I, int W, int X, int Y,
                     int* out_E, double* out_lng, int max_out, int* n_out);
 
-/* ln Z(beta) = log sum_E g(E) exp(-beta E).  No counterpart in the reference
- * (its output stops at ln g, README.md:198-200 cites the decoding step). */
+/* ln Z(beta) = log sum_E g(E) exp(-beta E); NaN when n > 0 and E or lng is
+ * null.  No counterpart in the reference (its output stops at ln g,
+ * README.md:198-200 cites the decoding step). */
 double wl_log_partition(const int* E, const double* lng, int n, double beta);
 
 /* Nishimori inverse temperature 1/2 ln((1-p)/p). */

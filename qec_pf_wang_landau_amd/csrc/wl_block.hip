@@ -712,6 +712,9 @@ int wl_block_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, 
     if (num_tasks == 0 || max_rounds <= 0) {
       return;
     }
+    require_buffers("block_run", {d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy, d_offset_iter,
+                                  d_offset_lattice, d_idx, d_cond, d_cond_sample, d_start, d_end, d_spectrum,
+                                  d_counter, d_rounds, d_status});
     hipStream_t user = reinterpret_cast<hipStream_t>(stream);
     int n_iv = 0;
     for (int q = 0; q < num_tasks; ++q) {

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 
@@ -49,6 +50,18 @@ inline void require(bool ok, const char* what)
 {
   if (!ok) {
     throw std::invalid_argument(what);
+  }
+}
+
+// Throws std::invalid_argument naming the entry point when a buffer the call
+// reads or writes is null.  Entry points check after their empty-call early
+// return, so a call with nothing to do needs no buffers.
+inline void require_buffers(const char* what, std::initializer_list<const void*> ptrs)
+{
+  for (const void* p : ptrs) {
+    if (p == nullptr) {
+      throw std::invalid_argument(std::string(what) + ": a required buffer is null");
+    }
   }
 }
 

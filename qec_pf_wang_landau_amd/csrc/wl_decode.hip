@@ -78,6 +78,8 @@ int wl_stitch_tasks(const wl_task* d_tasks, int num_tasks, const double* d_logG,
     if (num_tasks == 0) {
       return;
     }
+    require_buffers("stitch_tasks", {d_tasks, d_logG, d_start, d_end, d_cond_sample, d_beta, d_work, d_work_off,
+                                     d_out_E, d_out_lng, d_out_off, d_out_n, d_log_z});
     k_stitch_tasks<<<grid_for(num_tasks, 64), 64, 0, reinterpret_cast<hipStream_t>(stream)>>>(
       d_tasks, num_tasks, d_logG, d_start, d_end, d_cond_sample, d_beta, static_cast<unsigned char*>(d_work),
       d_work_off, d_out_E, d_out_lng, d_out_off, d_out_n, d_log_z);

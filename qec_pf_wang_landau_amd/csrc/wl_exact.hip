@@ -151,6 +151,7 @@ int wl_exact_dos(const int8_t* d_J, int X, int Y, int boundary, int num_samples,
     if (num_samples == 0) {
       return;
     }
+    require_buffers("exact_dos", {d_J, d_counts});
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int bins = 4 * N + 1;
     const int64_t total = static_cast<int64_t>(num_samples) * bins;

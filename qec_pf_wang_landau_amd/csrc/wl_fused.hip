@@ -637,6 +637,9 @@ int wl_fused_prepare(const wl_task* h_tasks, int num_tasks, const int* h_start, 
 {
   return guarded([&] {
     require(num_tasks >= 0 && h_tasks != nullptr && plan != nullptr, "fused_prepare: bad task table");
+    if (num_tasks > 0) {
+      require_buffers("fused_prepare", {h_start, h_end});
+    }
     int max_block = 64;
     size_t max_lds = 0;
     for (int q = 0; q < num_tasks; ++q) {
@@ -684,6 +687,10 @@ int wl_fused_launch(const wl_fused_launch_plan* plan, const wl_task* d_tasks, in
     if (plan->num_tasks == 0) {
       return;
     }
+    // d_task_ticks is optional
+    require_buffers("fused_launch", {d_tasks, d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy,
+                                     d_offset_iter, d_offset_lattice, d_idx, d_cond, d_cond_sample, d_start, d_end,
+                                     d_spectrum, d_counter, d_rounds, d_status});
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     task_state s{d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy, d_offset_iter, d_offset_lattice,
                  d_idx, d_cond, d_cond_sample, d_start, d_end, d_spectrum, d_counter, d_rounds, d_status,
@@ -711,6 +718,9 @@ int wl_fused_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, 
     if (num_tasks == 0) {
       return;
     }
+    require_buffers("fused_run", {d_tasks, d_lattice, d_interactions, d_H, d_logG, d_factor, d_energy, d_offset_iter,
+                                  d_offset_lattice, d_idx, d_cond, d_cond_sample, d_start, d_end, d_spectrum,
+                                  d_counter, d_rounds, d_status});
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     int n_iv = 0;
     for (int q = 0; q < num_tasks; ++q) {

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <numeric>
 #include <stdexcept>
@@ -38,6 +39,16 @@ int guarded_host(F&& fn)
   }
 }
 
+// A required argument that is null is an invalid argument (status 1).
+void require_args(const char* what, std::initializer_list<const void*> ptrs)
+{
+  for (const void* p : ptrs) {
+    if (p == nullptr) {
+      throw std::invalid_argument(std::string(what) + ": a required argument is null");
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -48,6 +59,7 @@ int wl_interval_results(int e_min, int e_max, unsigned int num_intervals, int nu
                         int* h_end, int64_t* len_histogram_over_all_walkers, int* len_interval)
 {
   return guarded_host([&] {
+    require_args("interval_results", {h_start, h_end, len_histogram_over_all_walkers, len_interval});
     if (overlap > 1 || overlap < 0) {
       throw std::invalid_argument("Overlap is out of bounds.");
     }
@@ -84,6 +96,7 @@ int wl_energy_spectrum(const uint8_t* hist, int len_hist, int X, int Y, int* E_m
     if (len_hist != 4 * X * Y + 1) {
       throw std::invalid_argument("histogram length does not match 4XY+1");
     }
+    require_args("energy_spectrum", {hist, E_min, E_max, mask, len_mask});
     int first = -1, last = -1;
     for (int e = 0; e < len_hist; ++e) {
       if (hist[e] == 1) {
@@ -106,9 +119,12 @@ int wl_energy_spectrum(const uint8_t* hist, int len_hist, int X, int Y, int* E_m
 }
 
 // Spacing of the energies present in a spectrum mask (gcd of their
-// differences; 0 for a single energy).
+// differences; 0 for a single energy or no mask).
 int wl_energy_stride(const int8_t* mask, int len)
 {
+  if (mask == nullptr) {
+    return 0;
+  }
   int g = 0, prev = -1;
   for (int e = 0; e < len; ++e) {
     if (mask[e] == 1) {
@@ -127,6 +143,12 @@ int wl_select_seed_lattices(const int8_t* found, const int* found_energy, int nu
                             const int* end, int I, int W, int8_t* out)
 {
   return guarded_host([&] {
+    if (num_found > 0) {
+      require_args("select_seed_lattices", {found, found_energy});
+    }
+    if (I > 0) {
+      require_args("select_seed_lattices", {start, end, out});
+    }
     std::map<int, int> by_energy;  // energy -> index of the found lattice
     for (int q = 0; q < num_found; ++q) {
       by_energy.emplace(found_energy[q], q);
@@ -158,6 +180,7 @@ int wl_stitch_log_g(const double* logG_sample, const int* start, const int* end,
     if (I < 1 || W < 1) {
       throw std::invalid_argument("stitch: bad shape");
     }
+    require_args("stitch_log_g", {logG_sample, start, end, out_E, out_lng, n_out});
     int total = 0;
     for (int i = 0; i < I; ++i) {
       total += end[i] - start[i] + 1;
@@ -182,6 +205,9 @@ int wl_stitch_log_g(const double* logG_sample, const int* start, const int* end,
 
 double wl_log_partition(const int* E, const double* lng, int n, double beta)
 {
+  if (n > 0 && (E == nullptr || lng == nullptr)) {
+    return std::nan("");
+  }
   return wlamd::log_partition(E, lng, n, beta);
 }
 

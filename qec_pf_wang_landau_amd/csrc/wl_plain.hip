@@ -625,6 +625,7 @@ int wl_calc_energy(int boundary, int* d_energy, const int8_t* d_lattice, const i
     if (num_lattices == 0) {
       return;
     }
+    require_buffers("calc_energy", {d_energy, d_lattice, d_interactions, d_offset_lattice});
     k_calc_energy<<<grid_for(num_lattices, 128), 128, 0, as_stream(stream)>>>(
       boundary, d_energy, d_lattice, d_interactions, d_offset_lattice, dim_X, dim_Y,
       static_cast<int64_t>(num_lattices), walker_per_interactions);
@@ -640,6 +641,7 @@ int wl_init_interactions(int8_t* d_interactions, size_t lattice_size, int num_in
     if (n == 0) {
       return;
     }
+    require_buffers("init_interactions", {d_interactions});
     k_init_interactions<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(
       d_interactions, static_cast<int64_t>(lattice_size), num_interactions, seed, prob);
     WL_LAUNCH_CHECK();
@@ -652,6 +654,7 @@ int wl_apply_x_horizontal_error(int8_t* d_interactions, int X, int Y, int S, voi
     if (X * Y * S == 0) {
       return;
     }
+    require_buffers("apply_x_horizontal_error", {d_interactions});
     k_apply_horizontal<<<grid_for(static_cast<int64_t>(Y) * S, 128), 128, 0, as_stream(stream)>>>(d_interactions, X,
                                                                                                   Y, S);
     WL_LAUNCH_CHECK();
@@ -664,6 +667,7 @@ int wl_apply_x_vertical_error(int8_t* d_interactions, int X, int Y, int S, void*
     if (X * Y * S == 0) {
       return;
     }
+    require_buffers("apply_x_vertical_error", {d_interactions});
     k_apply_vertical<<<grid_for(static_cast<int64_t>(X) * S, 128), 128, 0, as_stream(stream)>>>(d_interactions, X,
                                                                                                 Y, S);
     WL_LAUNCH_CHECK();
@@ -679,6 +683,7 @@ int wl_init_lattice(int8_t* d_lattice, float* d_probs, int X, int Y, size_t num_
     if (n == 0) {
       return;
     }
+    require_buffers("init_lattice", {d_lattice});  // d_probs is optional
     k_init_lattice<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(d_lattice, d_probs, X, Y,
                                                                       static_cast<int64_t>(num_lattices), seed,
                                                                       walker_per_interactions);
@@ -692,6 +697,7 @@ int wl_init_offsets_lattice(int* d_off, int X, int Y, size_t num_lattices, void*
     if (num_lattices == 0) {
       return;
     }
+    require_buffers("init_offsets_lattice", {d_off});
     k_init_offsets_lattice<<<grid_for(num_lattices, 256), 256, 0, as_stream(stream)>>>(
       d_off, X * Y, static_cast<int64_t>(num_lattices));
     WL_LAUNCH_CHECK();
@@ -715,6 +721,8 @@ int wl_wang_landau_pre_run(int* d_energy, int* d_flag_found_interval, int8_t* d_
     if (S == 0) {
       return;
     }
+    require_buffers("wang_landau_pre_run", {d_energy, d_flag_found_interval, d_lattice, d_found_interval_lattice,
+                                            d_histogram, d_seed_offset, d_interactions, d_offset_lattice_per_walker});
     const int len_hist = E_max - E_min + 1;
     const size_t lds = sizeof(uint32_t) * len_hist + 2 * sizeof(int) * num_intervals;
     require(lds <= 64 * 1024, "pre_run: histogram does not fit the workgroup's LDS");
@@ -734,6 +742,7 @@ int wl_check_energy_ranges(int8_t* flags, const int* E, const int* start, const 
     if (total == 0) {
       return;
     }
+    require_buffers("check_energy_ranges", {flags, E, start, end});
     k_check_energy_ranges<<<grid_for(total, 256), 256, 0, as_stream(stream)>>>(flags, E, start, end,
                                                                                static_cast<int64_t>(total), W);
     WL_LAUNCH_CHECK();
@@ -747,6 +756,7 @@ int wl_init_indices(int* idx, size_t total, int W, void* stream)
     if (total == 0) {
       return;
     }
+    require_buffers("init_indices", {idx});
     k_init_indices<<<grid_for(total, 256), 256, 0, as_stream(stream)>>>(idx, static_cast<int64_t>(total), W);
     WL_LAUNCH_CHECK();
   });
@@ -758,6 +768,7 @@ int wl_reset_d_cond(int8_t* cond, const double* factor, int total_intervals, dou
     if (total_intervals <= 0) {
       return;
     }
+    require_buffers("reset_d_cond", {cond, factor});
     k_reset_d_cond<<<grid_for(total_intervals, 256), 256, 0, as_stream(stream)>>>(cond, factor, total_intervals, beta,
                                                                                   W);
     WL_LAUNCH_CHECK();
@@ -772,6 +783,7 @@ int wl_init_offsets_histogram(int* off, const int* start, const int* end, const 
     if (total_walker <= 0) {
       return;
     }
+    require_buffers("init_offsets_histogram", {off, start, end, len_hist});
     k_init_offsets_histogram<<<grid_for(total_walker, 256), 256, 0, as_stream(stream)>>>(off, start, end, len_hist, I,
                                                                                          total_walker, W);
     WL_LAUNCH_CHECK();
@@ -790,6 +802,8 @@ int wl_calc_average_log_g(double* d_shared_logG, const int* d_len_histograms, co
     if (S <= 0) {
       return;
     }
+    require_buffers("calc_average_log_g", {d_shared_logG, d_log_G, d_start, d_end, d_spectrum, d_cond, d_off_hist,
+                                           d_off_spec, d_off_shared, d_cond_sample});
     k_calc_average<<<S * I, 64, 0, as_stream(stream)>>>(d_shared_logG, d_log_G, d_start, d_end, d_spectrum, d_cond,
                                                         d_off_hist, d_off_spec, d_off_shared, d_cond_sample, W, I);
     WL_LAUNCH_CHECK();
@@ -803,6 +817,7 @@ int wl_check_interactions_finished(int* d_cond_sample, const int8_t* d_cond, con
     if (S <= 0) {
       return;
     }
+    require_buffers("check_interactions_finished", {d_cond_sample, d_cond, d_off_intervals});
     k_finished<<<grid_for(S, 256), 256, 0, as_stream(stream)>>>(d_cond_sample, d_cond, d_off_intervals, I, S);
     WL_LAUNCH_CHECK();
   });
@@ -819,6 +834,8 @@ int wl_replica_exchange(int* d_offset_lattice, int* d_energy, int* d_idx, uint64
     if (S <= 0 || I < 2) {
       return;
     }
+    require_buffers("replica_exchange", {d_offset_lattice, d_energy, d_idx, d_seed_offset, d_start, d_end, d_off_hist,
+                                         d_cond_sample, d_logG});
     k_replica_exchange<<<S * I, round_up64(W), 0, as_stream(stream)>>>(d_offset_lattice, d_energy, d_idx,
                                                                        d_seed_offset, d_start, d_end, d_off_hist,
                                                                        d_cond_sample, d_logG, even, seed, I,
@@ -838,6 +855,8 @@ int wl_check_histogram(uint64_t* d_H, double* d_factor, int8_t* d_cond, const in
     if (total <= 0) {
       return;
     }
+    require_buffers("check_histogram", {d_H, d_factor, d_cond, d_off_hist, d_end, d_start, d_spectrum, d_off_spec,
+                                        d_cond_sample});
     k_check_histogram<<<total / W, round_up64(W), 0, as_stream(stream)>>>(d_H, d_factor, d_cond, d_off_hist, d_end,
                                                                          d_start, d_spectrum, d_off_spec,
                                                                          d_cond_sample, alpha, W, I);
@@ -856,6 +875,8 @@ int wl_redistribute_g_values(double* d_log_G, const double* d_shared, const int*
     if (S <= 0) {
       return;
     }
+    require_buffers("redistribute_g_values", {d_log_G, d_shared, d_start, d_end, d_cond_sample, d_off_hist, d_cond,
+                                              d_off_shared});
     k_redistribute<<<S * I, 256, 0, as_stream(stream)>>>(d_log_G, d_shared, d_start, d_end, d_cond_sample, d_off_hist,
                                                          d_cond, d_off_shared, I, W);
     WL_LAUNCH_CHECK();
@@ -874,6 +895,9 @@ int wl_wang_landau(int8_t* d_lattice, const int8_t* d_J, uint64_t* d_H, double* 
     if (num_lattices <= 0) {
       return;
     }
+    require_buffers("wang_landau", {d_lattice, d_J, d_H, d_logG, factor, d_energy, d_newE, d_off_iter, foundFlag,
+                                    d_start, d_end, d_off_hist, d_off_lattice, d_cond, d_spectrum, d_off_spec,
+                                    d_cond_sample});
     k_wang_landau<<<grid_for(num_lattices, kBlock), kBlock, 0, as_stream(stream)>>>(
       d_lattice, d_J, d_H, d_logG, factor, d_energy, d_newE, d_off_iter, foundFlag, d_start, d_end, d_off_hist,
       d_off_lattice, d_cond, d_spectrum, d_off_spec, d_cond_sample, X, Y, num_iterations, num_lattices, I, per_sample,
@@ -888,6 +912,7 @@ int wl_accept_decisions(const double* d_delta, const uint32_t* d_words, int n, i
     if (n <= 0) {
       return;
     }
+    require_buffers("accept_decisions", {d_delta, d_words, d_out});
     k_accept<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(d_delta, d_words, n, d_out);
     WL_LAUNCH_CHECK();
   });
@@ -901,6 +926,7 @@ int wl_fisher_yates(int* d_shuffle, uint64_t* d_seed_offset, int seed, int total
     if (total_intervals <= 0) {
       return;
     }
+    require_buffers("fisher_yates", {d_shuffle, d_seed_offset});
     k_fisher_yates<<<grid_for(total_intervals, 64), 64, 0, as_stream(stream)>>>(d_shuffle, d_seed_offset, seed,
                                                                                 total_intervals, W, per_sample);
     WL_LAUNCH_CHECK();
@@ -917,6 +943,8 @@ int wl_mainrun_loop(int8_t* d_lattice, const int8_t* d_J, uint64_t* d_H, double*
   int status = 0;
   const int rc = guarded([&] {
     require(S > 0 && I > 0 && W > 0 && exchange_every > 0, "mainrun_loop: bad shape");
+    require_buffers("mainrun_loop", {d_lattice, d_J, d_H, d_logG, d_factor, d_energy, d_off_iter, d_off_lattice, d_idx,
+                                     d_cond, d_cond_sample, d_start, d_end, d_off_hist, d_spectrum, d_off_spec});
     hipStream_t st = as_stream(stream);
     const int total_walker = S * I * W;
     const int per_sample = I * W;

@@ -49,6 +49,35 @@ def test_errors_are_status_codes_not_exceptions():
         engine.interval_results(-16, 16, 64, 8, 1.0)
 
 
+def test_null_buffers_are_invalid_arguments():
+    """Entry points refuse a null buffer they would read or write with status
+    1 before touching the device (so this runs without a GPU); calls with
+    nothing to do accept null buffers."""
+    lib = _lib.lib()
+    null = None
+    cases = [
+        ("wl_calc_energy", (0, null, null, null, null, 4, 4, 16, 16, null)),
+        ("wl_init_interactions", (null, 16, 2, 1, C.c_float(0.1), null)),
+        ("wl_init_indices", (null, 64, 8, null)),
+        ("wl_reset_d_cond", (null, null, 4, C.c_double(1e-6), 8, null)),
+        ("wl_check_interactions_finished", (null, null, null, 4, 2, null)),
+        ("wl_accept_decisions", (null, null, 10, null, null)),
+        ("wl_exact_dos", (null, 4, 4, 0, 1, null, null)),
+    ]
+    for name, args in cases:
+        assert getattr(lib, name)(*args) == 1, name
+        assert "null" in _lib.last_error(), (name, _lib.last_error())
+    # nothing to do: null buffers are fine
+    assert lib.wl_calc_energy(0, null, null, null, null, 4, 4, 0, 16, null) == 0
+    assert lib.wl_init_indices(null, 0, 8, null) == 0
+    assert lib.wl_exact_dos(null, 4, 4, 0, 0, null, null) == 0
+    st = np.zeros(4, np.int32)
+    assert lib.wl_interval_results(-16, 16, 4, 8, C.c_float(0.25), st.ctypes.data, null, null, null) == 1
+    assert lib.wl_stitch_log_g(null, null, null, 2, 2, 4, 4, null, null, 10, null) == 1
+    assert lib.wl_energy_stride(null, 10) == 0
+    assert math.isnan(lib.wl_log_partition(null, null, 3, C.c_double(0.5)))
+
+
 def test_interval_partition_matches_oracle_and_reference():
     r = ref()
     rng = np.random.default_rng(1)
