@@ -1,5 +1,6 @@
-"""Edge cases of the batched forms: empty task tables, the widest lattice
-rows (Y = 32 fills every bit of a row word), a large interval (600 walkers,
+"""Edge cases of the batched forms: empty task tables, row widths at the
+limits of the LDS row layouts (Y = 32 fills every bit of a row word, Y = 16
+is the widest row held twice), a large interval (600 walkers,
 the per-interval path's 1024-thread variant), and inputs the
 kernels must refuse instead of indexing outside their LDS (walker energy
 off the interval, histogram count beyond 32 bits)."""
@@ -36,11 +37,12 @@ def test_empty_task_tables(cuda):
 
 
 @pytest.mark.parametrize("boundary", [0, 1])
-@pytest.mark.parametrize("Y", [32, 17, 16])
+@pytest.mark.parametrize("Y", [32, 17, 16, 7])
 def test_widest_rows_both_forms(cuda, boundary, Y):
     """Row widths at the two LDS row layouts' limits: Y = 32 fills one u32
     word, Y = 17 is the narrowest row held once (the right neighbour of the
-    last column wraps by a shift), Y = 16 the widest row held twice."""
+    last column wraps by a shift), Y = 16 the widest row held twice; Y = 7
+    an odd width on a non-square lattice."""
     cfg = engine.RunConfig(X=4, Y=Y, boundary=boundary, num_intervals=3, walker_per_interval=4, overlap=0.25,
                            alpha=0.8, beta=1e-6, num_iterations=200, exchange_every=3, seed=12345)
     st = engine.build_state(oracle_samples(cfg, 1, 0.1, "h", prerun_walkers=64, prerun_iters=20000), cfg)
