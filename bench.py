@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-peaks", action="store_true",
                     help="skip measuring the LDS read and HBM copy bandwidth after the timed steps")
-    ap.add_argument("--cpu-tasks", type=int, default=16)
+    ap.add_argument("--cpu-tasks", type=int, default=48)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after timing, run one device copy of this many MiB (the reference kernel that scales "
                          "the HBM byte counters)")
