@@ -82,6 +82,17 @@ def test_fast_acceptance_equals_exact_rule(cuda):
     # d within a few 2^-14 of ln u: both edges of the FP32 band and its inside
     lu = np.log((x[200000:300000].astype(np.float64) + 0.5) * 2.0**-32)
     d[200000:300000] = lu + rng.uniform(-4e-4, 4e-4, lu.size)
+    # d rounded to FP32 for the fast compare: values whose FP32 rounding
+    # overflows, underflows to a subnormal or zero, or sits next to the
+    # -2^-126 limit, each against the extreme words and random ones
+    special = np.array([1e300, 3.5e38, -3.5e38, -1e300, 1e-40, -1e-40, -1e-300, -2.0**-126, -2.0**-126 * 1.5,
+                        -2.0**-127, -22.18070977791825, -22.873, -23.0, -5e-5, -6.2e-5, -6.0e-5])
+    words = np.array([0, 1, 2**31, 2**32 - 2, 2**32 - 1], np.uint32)
+    sd = np.repeat(special, words.size + 4)
+    sx = np.concatenate([np.concatenate([words, rng.integers(0, 2**32, 4, dtype=np.uint64).astype(np.uint32)])
+                         for _ in special])
+    d = np.concatenate([d, sd])
+    x = np.concatenate([x, sx])
     o = oracle()
     expect = np.array([1 if (float(xi) + 0.5) / 2**32 < (1.0 if di >= 0 else o.wlo_exp(float(di))) else 0
                        for di, xi in zip(d, x)], np.int8)
