@@ -513,10 +513,18 @@ __global__ void k_block_plan(const wl_task* tasks, int num_tasks, blk_state st, 
   }
 }
 
+// Index and swap-target buffers of the exchange shuffle (ints each; one
+// interval needs W + 1 <= 1025).
+constexpr int kShuffleWords = 4096;
+
+// Threads of the per-task exchange workgroup: each walker pair's acceptance
+// is a chain of dependent HBM reads, so many pairs in flight hide it.
+constexpr int kEndThreads = 1024;
+
 // Per task after the block: finish / abort / error bookkeeping, then the
 // replica exchange (even pass, odd pass) when the block ended on an
 // exchange round.  Walker state is read and written in HBM.
-__global__ void __launch_bounds__(256) k_block_end(const wl_task* tasks, blk_state st, blk_scratch sc)
+__global__ void __launch_bounds__(kEndThreads) k_block_end(const wl_task* tasks, blk_state st, blk_scratch sc)
 {
   const int q = blockIdx.x;
   const int L = sc.task_len[q];
@@ -527,6 +535,8 @@ __global__ void __launch_bounds__(256) k_block_end(const wl_task* tasks, blk_sta
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   __shared__ int s_err, s_abort, s_final, s_rstar, s_sum;
+  __shared__ int s_perm[kShuffleWords];
+  __shared__ int s_pick[kShuffleWords];
   if (tid == 0) {
     int err = 0, abort = INT_MAX, fin = 1, rstar = 0, sum = 0;
     for (int kk = 0; kk < t.I; ++kk) {
@@ -585,25 +595,54 @@ __global__ void __launch_bounds__(256) k_block_end(const wl_task* tasks, blk_sta
     const int len0 = g_end[0] - g_start[0] + 1;
     for (int pass = 0; pass < 2; ++pass) {
       const bool even = pass == 0;
-      for (int kk = tid; kk < I - 1; kk += nthr) {
-        const int par = (compat ? (t.parity_base + kk) : kk) & 1;
-        if (even ? par != 0 : par == 0) {
-          continue;
+      // Fisher-Yates shuffle of each exchanging interval's index segment
+      // (src/wl.cu:785-808): the W - 1 swap targets come from words
+      // d0 .. d0 + W - 2 of the interval's walker-0 exchange stream, which
+      // are independent draws of a counter-based generator, so the lanes
+      // compute them side by side; the swaps themselves run in order, one
+      // lane per interval, on a copy of the segment in LDS.  Intervals go
+      // through in chunks that fit the LDS buffers.
+      const int first = (compat ? ((t.parity_base & 1) == (even ? 0 : 1) ? 0 : 1) : (even ? 0 : 1));
+      const int n_ex = (I - 1 - first + 1) / 2;  // intervals kk = first, first + 2, ... < I - 1
+      const int pitch = W + 1;  // odd pitch: the per-interval swap lanes hit different banks
+      const int per_chunk = max(1, kShuffleWords / pitch);
+      for (int c0i = 0; c0i < n_ex; c0i += per_chunk) {
+        const int nc = min(per_chunk, n_ex - c0i);
+        for (int x = tid; x < nc * W; x += nthr) {
+          const int ci = x / W;
+          const int j = x - ci * W;
+          const int kk = first + 2 * (c0i + ci);
+          s_perm[ci * pitch + j] = st.idx[wb + static_cast<int64_t>(kk) * W + j];
+          if (j < W - 1) {
+            const uint64_t d0 = st.off_iter[wb + static_cast<int64_t>(kk) * W] >> kAttemptBits;
+            const uint32_t word = exchange_word(t.rng_key, static_cast<uint32_t>(kk * W), d0 + j);
+            s_pick[ci * pitch + j] = pick_index(word, static_cast<uint32_t>(W - j));  // target for ii = W - 1 - j
+          }
         }
-        int* seg = st.idx + wb + static_cast<int64_t>(kk) * W;
-        uint64_t* oi = st.off_iter + wb + static_cast<int64_t>(kk) * W;
-        const uint64_t c0 = *oi;
-        stream_reader xr;
-        xr.init(t.rng_key, kTagExchange, static_cast<uint32_t>(kk * W), c0 >> kAttemptBits);
-        for (int ii = W - 1; ii > 0; --ii) {
-          const int r = pick_index(xr.next_u32(), static_cast<uint32_t>(ii + 1));
-          const int tmp = seg[ii];
-          seg[ii] = seg[r];
-          seg[r] = tmp;
+        __syncthreads();
+        for (int ci = tid; ci < nc; ci += nthr) {
+          int* perm = s_perm + ci * pitch;
+          const int* pick = s_pick + ci * pitch;
+          for (int ii = W - 1; ii > 0; --ii) {
+            const int r = pick[W - 1 - ii];
+            const int tmp = perm[ii];
+            perm[ii] = perm[r];
+            perm[r] = tmp;
+          }
+          const int kk = first + 2 * (c0i + ci);
+          uint64_t* oi = st.off_iter + wb + static_cast<int64_t>(kk) * W;
+          const uint64_t cw = *oi;
+          *oi = (cw & kAttemptMask) | (((cw >> kAttemptBits) + static_cast<uint64_t>(W - 1)) << kAttemptBits);
         }
-        *oi = (c0 & kAttemptMask) | (xr.draw << kAttemptBits);
+        __syncthreads();
+        for (int x = tid; x < nc * W; x += nthr) {
+          const int ci = x / W;
+          const int j = x - ci * W;
+          const int kk = first + 2 * (c0i + ci);
+          st.idx[wb + static_cast<int64_t>(kk) * W + j] = s_perm[ci * pitch + j];
+        }
+        __syncthreads();
       }
-      __syncthreads();
       for (int l = tid; l < (I - 1) * W; l += nthr) {
         const int kk = l / W;
         const int w = l - kk * W;
@@ -829,7 +868,7 @@ int wl_block_run(const wl_task* d_tasks, const wl_task* h_tasks, int num_tasks, 
           launch_intervals<1024>(false, n_iv, max_lds, q, d_tasks, s, sc);
           launch_intervals<1024>(true, n_iv, max_lds, q, d_tasks, s, sc);
         }
-        k_block_end<<<num_tasks, 256, 0, q>>>(d_tasks, s, sc);
+        k_block_end<<<num_tasks, kEndThreads, 0, q>>>(d_tasks, s, sc);
       };
       hipStream_t run = user;
       if (use_graph) {
