@@ -385,6 +385,7 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
     // them against lane 0's update below
     const int cond0 = misc[2];
     const double f0 = *fptr;
+    bool flat_walker = false;
     if (walker && cond0 == 0) {
       const uint32_t* Hw = Hc + tid;
       uint32_t mn = 0xffffffffu;
@@ -400,11 +401,10 @@ __global__ void __launch_bounds__(MAXB) k_interval(const wl_task* tasks, blk_sta
       }
       if (cnt > 0) {
         const double mean = static_cast<double>(sum) / cnt;
-        if (static_cast<double>(mn) >= t.alpha * mean) {
-          atomicAdd(&misc[1], 1);
-        }
+        flat_walker = static_cast<double>(mn) >= t.alpha * mean;
       }
     }
+    vote_flat(flat_walker, tid, W, misc + 1);  // all walkers are interval 0 of this workgroup: misc[1]
     __syncthreads();
     const bool flat = cond0 == 0 && misc[1] == W;
     if (flat && walker) {

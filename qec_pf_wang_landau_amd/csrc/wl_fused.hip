@@ -392,6 +392,7 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       break;
     }
     // ---- flatness (src/wl.cu:892-967) --------------------------------
+    bool flat_walker = false;
     if (walker && Lcond[k] == 0) {
       const int e0 = Le0[k], nb = Lnb[k];
       const uint32_t* Hw = Hc + Lgb[k] + w;
@@ -408,11 +409,10 @@ __device__ __forceinline__ void run_task(const wl_task& t, const task_state& st,
       }
       if (cnt > 0) {
         const double mean = static_cast<double>(sum) / cnt;
-        if (static_cast<double>(mn) >= t.alpha * mean) {
-          atomicAdd(&Lvote[k], 1);
-        }
+        flat_walker = static_cast<double>(mn) >= t.alpha * mean;
       }
     }
+    vote_flat(flat_walker, tid, W, Lvote);
     __syncthreads();
     if (walker && Lcond[k] == 0 && Lvote[k] == W) {
       uint32_t* Hw = Hc + Lgb[k] + w;

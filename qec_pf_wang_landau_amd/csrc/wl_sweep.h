@@ -133,6 +133,27 @@ __device__ __forceinline__ int mul_i24(int a, int b)
   return r;
 }
 
+// Flatness votes of a workgroup whose walker t (= thread t) belongs to
+// interval t / W: one ballot per wave, and the first lane of each interval's
+// run of lanes in the wave adds that run's count of flat walkers to
+// votes[interval], so an interval takes at most two LDS atomics instead of
+// one per walker.  Every lane of the workgroup calls this (lanes that are
+// not walkers, or walkers of converged intervals, pass flat = false).
+__device__ __forceinline__ void vote_flat(bool flat, int tid, int W, int* votes)
+{
+  const uint64_t m = __builtin_amdgcn_ballot_w64(flat);
+  const int lane = tid & 63;
+  const int pos = tid % W;  // position of this lane's walker in its interval
+  if (m != 0 && (pos == 0 || lane == 0)) {
+    const int len = min(W - pos, 64 - lane);  // lanes of this interval from here to the wave's end
+    const uint64_t run = (len >= 64 ? ~0ull : ((1ull << len) - 1ull)) << lane;
+    const int c = __popcll(m & run);
+    if (c > 0) {
+      atomicAdd(&votes[tid / W], c);
+    }
+  }
+}
+
 // Everything an attempt reads from LDS, issued before any of it is used.
 struct attempt_reads {
   int i, j, jl, u1, u2;
