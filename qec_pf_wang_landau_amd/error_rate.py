@@ -239,13 +239,14 @@ def decode_result_files(p: float, files: list[str | Path], z: float = 1.96, bina
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="Logical error rates from Wang-Landau decoding on the GPU")
-    ap.add_argument("--L", type=int, nargs="+", default=[4, 6])
-    ap.add_argument("--p", type=float, nargs="+", default=[0.04, 0.08, 0.12])
-    ap.add_argument("--samples", type=int, default=100)
-    ap.add_argument("--seed", type=int, default=12345)
-    ap.add_argument("--boundary", type=int, choices=(0, 1), default=0)
+    ap.add_argument("--L", type=int, nargs="+", default=[4, 6], help="lattice sizes (L x L)")
+    ap.add_argument("--p", type=float, nargs="+", default=[0.04, 0.08, 0.12], help="bit-flip probabilities")
+    ap.add_argument("--samples", type=int, default=100, help="disorder samples per (L, p) point")
+    ap.add_argument("--seed", type=int, default=12345, help="seed of sample 0 (sample q uses seed + q)")
+    ap.add_argument("--boundary", type=int, choices=(0, 1), default=0, help="0 periodic, 1 open")
     ap.add_argument("--beta", type=float, default=1e-6, help="final ln f")
-    ap.add_argument("--method", choices=("wl", "exact"), default="wl")
+    ap.add_argument("--method", choices=("wl", "exact"), default="wl",
+                    help="Wang-Landau or exact enumeration (N <= 40)")
     args = ap.parse_args(argv)
     for L in args.L:
         cfg = run_config(L, boundary=args.boundary, seed=args.seed, beta=args.beta)
