@@ -74,3 +74,16 @@ def test_fused_matches_plain_loop(cuda):
     fused = dev.pull().split()[0]
     for k in ("lattice", "E", "off_iter", "logG", "H", "factor", "idx", "cond", "cond_sample", "off_lattice"):
         assert np.array_equal(getattr(plain, k), getattr(fused, k)), k
+
+
+def test_fused_intervals_crossing_wavefronts_bit_exact(cuda):
+    """12 intervals x 6 walkers: interval 10 spans lanes 60-65, across the
+    first wavefront's end, so its flatness vote comes from two waves; the
+    run is long enough for intervals to become flat and halve ln f."""
+    st = make_state(6, 6, 0, 12, 6, 1, 0.04, "i", n=400, c=5)
+    refs = oracle_per_task(st, 150)
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    dev.run(150)
+    got = dev.pull().split()[0]
+    compare(got, refs)
+    assert (got.cond != 0).any() or (got.factor < np.e).any(), "no interval became flat: the vote went untested"
