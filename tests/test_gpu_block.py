@@ -122,3 +122,18 @@ def test_mixed_run_and_autotuner_bit_exact(cuda):
         run.run(50)
         for got, rf in zip(run.pull(), refs):
             compare_tasks(got, rf)
+
+
+@pytest.mark.parametrize("W", [1, 2])
+def test_block_fewest_walkers_exchange_bit_exact(cuda, W):
+    """One or two walkers per interval with an exchange every round: the
+    shuffle has no draw (W = 1) or one; both batched forms against the
+    oracle."""
+    st = make_state(4, 4, 0, 3, W, 2, 0.1, "v", n=200, c=1)
+    refs = oracle_per_task(st, 40)
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    dev.run_blocks(40)
+    compare_tasks(dev.pull().split()[0], refs)
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    dev.run(40)
+    compare_tasks(dev.pull().split()[0], refs)
