@@ -55,6 +55,26 @@ def test_widest_rows_both_forms(cuda, boundary, Y):
     compare_tasks(dev.pull().split()[0], refs)
 
 
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_rows_held_once_to_convergence(cuda, boundary):
+    """A 3 x 20 lattice (rows held once in LDS) run to ln f <= 1e-3 on both
+    batched forms: flatness, averaging and exchanges all happen on that row
+    layout, and the final state equals the oracle's."""
+    cfg = engine.RunConfig(X=3, Y=20, boundary=boundary, num_intervals=3, walker_per_interval=4, overlap=0.25,
+                           alpha=0.8, beta=1e-3, num_iterations=200, exchange_every=3, seed=12345)
+    st = engine.build_state(oracle_samples(cfg, 1, 0.1, "h", prerun_walkers=64, prerun_iters=20000), cfg)
+    refs = oracle_per_task(st, 1 << 40)
+    assert refs[0].cond_sample[0] == -1
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    dev.run()
+    got = dev.pull().split()[0]
+    compare_tasks(got, refs)
+    assert got.status[0] == 1
+    dev = engine.BatchDevice(engine.merge_states([st]))
+    dev.run_blocks()
+    compare_tasks(dev.pull().split()[0], refs)
+
+
 def test_largest_interval_block_path(cuda):
     """600 walkers per interval (not a multiple of the wavefront): the
     1024-thread variant of the per-interval kernel, two intervals, exchange
